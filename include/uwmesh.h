@@ -1,0 +1,146 @@
+/*
+ * uwmesh.h -- C ABI of the MI355X (gfx950) chunk-meshing engine.
+ *
+ * The engine replaces the reference world generator's chunk builder: 3D multi-octave Perlin
+ * density sampled over each chunk's voxel grid, then marching-cubes isosurface extraction with
+ * the reference's vertex dedup. A chunk coordinate goes in; compact xyz-f32 vertex and u32 index
+ * buffers come out; blank chunks are flagged and skipped.
+ *
+ * Reference interfaces replaced (underwater_world/src/...):
+ *   noise::Perlin::new(seed)                    state.rs:358-359    -> uw_ctx_create
+ *   perlin_util::iso_at / perlin_3d_octaves     perlin_util.rs:6-29 -> uw_sample_density
+ *   Chunk::new + build_full / build_partial     chunk.rs:89-103,266-313 -> uw_chunk_build_host
+ *   Chunk::not_blank / num_inds / *_buffer_slice chunk.rs:344-348   -> uw_chunk_info fields
+ *   World::build_full_step / build_step loop    world.rs:113-145    -> uw_build_chunks_plain
+ *
+ * Conventions
+ *   - Every entry point returns an int status (UW_OK = 0, negative on error); the context keeps a
+ *     message for the last error (uw_last_error).
+ *   - Pointers documented "device" are HIP device pointers; "host" pointers are ordinary memory.
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream). Device-pointer entry
+ *     points are asynchronous on that stream; host-pointer entry points are synchronous.
+ *   - One context per host thread (or per stream); a context is not thread-safe.
+ *   - Vertex positions are world-space (chunk.rs:224-229); indices are chunk-local, three per
+ *     triangle, in the reference's table order (same winding). Vertex numbering follows the
+ *     reference's first-seen order over ordered corner pairs (chunk.rs:233-243).
+ */
+#ifndef UWMESH_H
+#define UWMESH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UW_MAX_OCTAVES 16
+#define UW_MAX_RESOLUTION 64 /* cells per chunk axis accepted by the chunk paths */
+
+enum uw_status {
+    UW_OK = 0,
+    UW_ERR_INVALID_ARG = -1,
+    UW_ERR_OVERFLOW = -2, /* output capacity too small; totals report the required sizes */
+    UW_ERR_HIP = -3,
+    UW_ERR_NO_DEVICE = -4,
+    UW_ERR_OOM = -5,
+};
+
+enum uw_bias_kind {
+    UW_BIAS_TERRACE = 0, /* rho = (adj + p) - fmod(adj, c), adj = (z * a) / b  (perlin_util.rs:27-28) */
+    UW_BIAS_LINEAR = 1,  /* rho = (z * a + p) + b */
+};
+
+/* Density-field and chunk-geometry parameters. uw_params_reference() fills the reference
+ * crate's constants (chunk.rs:5-12, perlin_util.rs:6-29). */
+typedef struct uw_params {
+    uint32_t seed;                        /* Perlin permutation seed (noise::Perlin::new)   */
+    int32_t octaves;                      /* 1..UW_MAX_OCTAVES  (chunk.rs:9)                */
+    double frequency;                     /* noise units per world unit (chunk.rs:108: 1/16) */
+    double lacunarity;                    /* octave frequency multiplier (perlin_util.rs:18) */
+    double amplitudes[UW_MAX_OCTAVES];    /* octave amplitudes (perlin_util.rs:17)           */
+    float iso;                            /* isovalue (chunk.rs:10)                          */
+    int32_t bias_kind;                    /* enum uw_bias_kind                               */
+    float bias_a, bias_b, bias_c;         /* bias coefficients, see enum uw_bias_kind        */
+    int32_t chunk_extent;                 /* world units per chunk edge (chunk.rs:5)         */
+} uw_params;
+
+/* Per-chunk result record (32 bytes). */
+enum uw_chunk_flags {
+    UW_CHUNK_BLANK = 1,      /* no triangles: the reference's !not_blank() (chunk.rs:291,344) */
+    UW_CHUNK_EARLY_SKIP = 2, /* every sample above iso: early_blank_check (chunk.rs:131-133)  */
+    UW_CHUNK_OVERFLOW = 4,   /* did not fit the output capacity; nothing written for it       */
+};
+
+typedef struct uw_chunk_info {
+    uint32_t flags;    /* enum uw_chunk_flags                                   */
+    uint32_t n_verts;  /* vertices of this chunk (num verts of chunk.rs:36)     */
+    uint32_t n_inds;   /* indices of this chunk (num_inds, chunk.rs:348)        */
+    uint32_t n_active; /* cells whose case is neither 0 nor 255                 */
+    uint64_t vert_off; /* first vertex of this chunk in uw_mesh_out.verts       */
+    uint64_t ind_off;  /* first index of this chunk in uw_mesh_out.inds         */
+} uw_chunk_info;
+
+/* Batch totals (device, written by the build): required sizes even on overflow. */
+typedef struct uw_totals {
+    uint64_t n_verts;
+    uint64_t n_inds;
+    uint32_t n_nonblank;
+    uint32_t overflow; /* 1 if any chunk overflowed */
+} uw_totals;
+
+/* Output description for the multi-chunk device entry points. All pointers are device
+ * pointers: verts holds 3 floats per vertex (vert_cap vertices), inds ind_cap u32 values,
+ * info one record per chunk, totals one record. */
+typedef struct uw_mesh_out {
+    float* verts;
+    uint32_t* inds;
+    uw_chunk_info* info;
+    uw_totals* totals;
+    uint64_t vert_cap;
+    uint64_t ind_cap;
+} uw_mesh_out;
+
+typedef struct uw_ctx uw_ctx;
+
+/* The reference crate's constants, with the given seed. */
+void uw_params_reference(uw_params* p, uint32_t seed);
+
+/* Create a context on HIP device `device`: validates the parameters, builds the permutation
+ * table (noise::Perlin::new, state.rs:358-359) and uploads it with the marching-cubes tables. */
+int uw_ctx_create(const uw_params* p, int device, uw_ctx** out);
+void uw_ctx_destroy(uw_ctx* ctx);
+const char* uw_last_error(const uw_ctx* ctx);
+
+/* The context's 256-entry permutation table (host copy). */
+int uw_ctx_permutation(const uw_ctx* ctx, uint8_t* out256);
+
+/* Host-only: the permutation table noise::Perlin::new(seed) builds (no device needed). */
+void uw_permutation_table(uint32_t seed, uint8_t* out256);
+
+/* Upper bounds for one chunk of n cells per axis: vertices 5 n (n+1)^2, indices 15 n^3. */
+uint64_t uw_max_verts(int32_t n);
+uint64_t uw_max_inds(int32_t n);
+
+/* Density sampling only (iso_at over every chunk sample, chunk.rs:105-129):
+ * coords  device int32[3 * nchunks] chunk coordinates,
+ * dens    device float[nchunks * (n+1)^3], per chunk index x (n+1)^2 + y (n+1) + z,
+ * below   device uint32[nchunks] or NULL: 1 if some sample is not above iso (not early-blank). */
+int uw_sample_density(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, int32_t n, float* dens,
+                      uint32_t* below, void* stream);
+
+/* Plain path: every stage of the chunk builder as its own kernel launch over all chunks
+ * (density -> classify + compact -> offsets -> emit). Uniform resolution n. `cases` may be NULL
+ * or a device uint8[nchunks * n^3] that receives every cell's cube case (chunk.rs:155-162). */
+int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, int32_t n,
+                          const uw_mesh_out* out, uint8_t* cases, void* stream);
+
+/* Drop-in form of Chunk::new(pos) + build_full (chunk.rs:89-103,266-268): one chunk, host
+ * buffers, synchronous. verts receives 3 floats per vertex, inds u32 indices. On
+ * UW_ERR_OVERFLOW nothing is copied and info->n_verts / n_inds give the sizes needed. */
+int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, float* verts,
+                        uint64_t vert_cap, uint32_t* inds, uint64_t ind_cap, uw_chunk_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* UWMESH_H */

@@ -1,0 +1,575 @@
+/*
+ * uw_oracle.c -- CPU restatement of the reference chunk builder, used ONLY as a checker.
+ *
+ * TEST INFRASTRUCTURE. Nothing in the engine (underwaterworld_amd/, include/) links, loads or
+ * calls this file; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg do,
+ * and only to check or to time the reference algorithm beside the GPU path.
+ *
+ * It restates, operation for operation, the reference's hot path:
+ *   - noise::Perlin::new(seed) -> PermutationTable::new (crate `noise` 0.8.2, built on
+ *     rand 0.8.5 SliceRandom::shuffle and rand_xorshift 0.3 XorShiftRng). Neither crate is
+ *     vendored in the reference nor present on this machine; the steps below restate their
+ *     published algorithms (call site: underwater_world/src/state.rs:358-359).
+ *   - noise::Perlin::get([f64;3]) -> core::perlin::perlin_3d (same crate; call site
+ *     underwater_world/src/perlin_util.rs:13).
+ *   - perlin_util::perlin_3d_octaves / iso_at (underwater_world/src/perlin_util.rs:6-29).
+ *   - Chunk::build_iso + corner_to_iso_idx (underwater_world/src/chunk.rs:105-129,351-353).
+ *   - Chunk::early_blank_check (chunk.rs:131-133).
+ *   - Chunk::build_mesh classification, triangle-table walk, edge lerp and ordered-pair
+ *     first-seen vertex dedup (chunk.rs:135-264), with the triangulation table of
+ *     marching_table.rs:1-272.
+ *
+ * Two numeric variants:
+ *   (i)  reference-faithful: f64 noise and coordinates, f32 density, f32 edge lerp, evaluated
+ *        in the reference's operation order. Compile with -ffp-contract=off (Rust never
+ *        contracts a*b+c), no fast-math.
+ *   (ii) FP64: the same sample positions, but density, lerp and positions carried in double.
+ *
+ * Pinning (what checks each function; see tests/test_oracle.py):
+ *   uwo_xorshift_next  -- Marsaglia xor128 known answer (first output 3701687786 for the
+ *                         published seed 123456789/362436069/521288629/88675123).
+ *   uwo_gen_index      -- exhaustive uniformity of the accept set on a reduced word size
+ *                         (same widening-multiply rule), and range bounds.
+ *   uwo_perm           -- bijection and determinism. The table values for a given seed are
+ *                         parity unpinned: the crate cannot be built or fetched here.
+ *   uwo_perlin3        -- lattice zeros, lattice gradients equal the hashed gradient vector
+ *                         (brute-force hash), closed form on lattice axes, 256-periodicity,
+ *                         |n| <= 1, and an independent numpy restatement. The gradient-case
+ *                         order for hash & 15 in 12..15, the final clamp and the
+ *                         interpolation form are recalled from the crate, not verified
+ *                         against it: parity to the crate itself is unpinned.
+ *   uwo_density_*      -- lattice known answers (fbm == 0 at integer noise coordinates, so
+ *                         rho is exactly the bias term), and the numpy restatement.
+ *   uwo_mesh_*         -- all 256 single-cell sign patterns (triangle count = table row
+ *                         length / 3), dyadic edge-lerp closed forms, literal linear-search
+ *                         dedup vs hashed dedup, vertex count == number of distinct ordered
+ *                         corner pairs, and the numpy restatement on small grids.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../underwaterworld_amd/csrc/mc_table.h"
+
+#define UWO_MAX_OCTAVES 16
+
+/* Parameters of the density field and the chunk geometry (mirrors the engine's uw_params
+ * field-for-field so that the same values can be handed to both). */
+typedef struct uwo_params {
+    uint32_t seed;
+    int32_t octaves;
+    double frequency;   /* noise units per world unit (reference: 1/16, chunk.rs:108) */
+    double lacunarity;  /* per-octave frequency multiplier (perlin_util.rs:18)        */
+    double amplitudes[UWO_MAX_OCTAVES]; /* per-octave amplitude (perlin_util.rs:17)  */
+    float iso;          /* isovalue (chunk.rs:10)                                      */
+    int32_t bias_kind;  /* 0 = terrace (perlin_util.rs:27-28), 1 = linear             */
+    float bias_a;       /* terrace: multiplier (16.0); linear: slope                   */
+    float bias_b;       /* terrace: divisor (32.0);    linear: offset                  */
+    float bias_c;       /* terrace: modulus (0.25)                                     */
+    int32_t chunk_extent; /* world units per chunk edge (chunk.rs:5)                   */
+} uwo_params;
+
+/* ------------------------------------------------------------------------------------- */
+/* rand_xorshift 0.3: XorShiftRng::from_seed / next_u32                                    */
+
+typedef struct { uint32_t x, y, z, w; } uwo_xorshift;
+
+uint32_t uwo_xorshift_next(uwo_xorshift* r) {
+    uint32_t x = r->x;
+    uint32_t t = x ^ (x << 11);
+    r->x = r->y;
+    r->y = r->z;
+    r->z = r->w;
+    uint32_t w = r->w;
+    r->w = w ^ (w >> 19) ^ (t ^ (t >> 8));
+    return r->w;
+}
+
+/* Exposed for the known-answer test: state -> n outputs. */
+void uwo_xorshift_stream(uint32_t x, uint32_t y, uint32_t z, uint32_t w, int n, uint32_t* out) {
+    uwo_xorshift r = {x, y, z, w};
+    for (int i = 0; i < n; ++i) out[i] = uwo_xorshift_next(&r);
+}
+
+/* rand 0.8.5 UniformInt<u32>::sample_single_inclusive(0, ubound-1) as used by gen_index:
+ * widening multiply with the conservative rejection zone (range << lz) - 1. */
+static uint32_t uwo_gen_index(uwo_xorshift* r, uint32_t ubound) {
+    uint32_t range = ubound;
+    uint32_t lz = (uint32_t)__builtin_clz(range);
+    uint32_t zone = (range << lz) - 1u;
+    for (;;) {
+        uint32_t v = uwo_xorshift_next(r);
+        uint64_t m = (uint64_t)v * (uint64_t)range;
+        uint32_t lo = (uint32_t)m;
+        if (lo <= zone) return (uint32_t)(m >> 32);
+    }
+}
+
+/* Same rule on 16-bit words, for the exhaustive uniformity check in the tests:
+ * returns the number of accepted v in [0, 2^16) mapping to each output in [0, range). */
+void uwo_gen_index_histogram16(uint32_t range, uint32_t* hist) {
+    uint32_t lz = (uint32_t)__builtin_clz(range) - 16u;
+    uint32_t zone = ((range << lz) - 1u) & 0xffffu;
+    for (uint32_t i = 0; i < range; ++i) hist[i] = 0;
+    for (uint32_t v = 0; v < 65536u; ++v) {
+        uint32_t m = v * range;
+        if ((m & 0xffffu) <= zone) hist[m >> 16]++;
+    }
+}
+
+/* noise 0.8.2 PermutationTable::new(seed): seed bytes [1,0,0,0, s,s,s] little-endian words
+ * x = 1, y = z = w = seed, then SliceRandom::shuffle of [0, 1, ..., 255]. */
+void uwo_perm(uint32_t seed, uint8_t* out) {
+    uwo_xorshift r = {1u, seed, seed, seed};
+    for (int i = 0; i < 256; ++i) out[i] = (uint8_t)i;
+    for (uint32_t i = 255; i >= 1; --i) {
+        uint32_t j = uwo_gen_index(&r, i + 1u);
+        uint8_t t = out[i];
+        out[i] = out[j];
+        out[j] = t;
+    }
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* noise 0.8.2 core::perlin::perlin_3d                                                     */
+
+static inline double uwo_grad_dot(uint32_t h, double x, double y, double z) {
+    switch (h & 15u) {
+        case 0: return x + y;
+        case 1: return -x + y;
+        case 2: return x - y;
+        case 3: return -x - y;
+        case 4: return x + z;
+        case 5: return -x + z;
+        case 6: return x - z;
+        case 7: return -x - z;
+        case 8: return y + z;
+        case 9: return -y + z;
+        case 10: return y - z;
+        case 11: return -y - z;
+        case 12: return x + y;
+        case 13: return -x + y;
+        case 14: return -y + z;
+        default: return -y - z;
+    }
+}
+
+static inline uint32_t uwo_hash3(const uint8_t* P, int64_t x, int64_t y, int64_t z) {
+    uint32_t a = P[(uint32_t)(x & 255)];
+    uint32_t b = P[a ^ (uint32_t)(y & 255)];
+    return P[b ^ (uint32_t)(z & 255)];
+}
+
+/* s_curve5 (interpolate.rs): t^3 (t (6t - 15) + 10) */
+static inline double uwo_quintic(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
+
+double uwo_perlin3(const uint8_t* P, double px, double py, double pz) {
+    const double SCALE_FACTOR = 1.1547005383792515; /* 1/(sqrt(3)/2) */
+    double fx = floor(px), fy = floor(py), fz = floor(pz);
+    int64_t cx = (int64_t)fx, cy = (int64_t)fy, cz = (int64_t)fz;
+    double dx = px - fx, dy = py - fy, dz = pz - fz;
+#define UWO_G(ox, oy, oz) \
+    uwo_grad_dot(uwo_hash3(P, cx + ox, cy + oy, cz + oz), dx - (double)ox, dy - (double)oy, dz - (double)oz)
+    double g000 = UWO_G(0, 0, 0);
+    double g100 = UWO_G(1, 0, 0);
+    double g010 = UWO_G(0, 1, 0);
+    double g110 = UWO_G(1, 1, 0);
+    double g001 = UWO_G(0, 0, 1);
+    double g101 = UWO_G(1, 0, 1);
+    double g011 = UWO_G(0, 1, 1);
+    double g111 = UWO_G(1, 1, 1);
+#undef UWO_G
+    double a = uwo_quintic(dx), b = uwo_quintic(dy), c = uwo_quintic(dz);
+    double k0 = g000;
+    double k1 = g100 - g000;
+    double k2 = g010 - g000;
+    double k3 = g001 - g000;
+    double k4 = g000 + g110 - g100 - g010;
+    double k5 = g000 + g101 - g100 - g001;
+    double k6 = g000 + g011 - g010 - g001;
+    double k7 = g100 + g010 + g001 + g111 - g000 - g110 - g101 - g011;
+    double r = k0 + k1 * a + k2 * b + k3 * c + k4 * a * b + k5 * a * c + k6 * b * c + k7 * a * b * c;
+    r = r * SCALE_FACTOR;
+    if (r < -1.0) r = -1.0;
+    if (r > 1.0) r = 1.0;
+    return r;
+}
+
+/* Batch form for the tests: n points (xyz interleaved) -> n values. */
+void uwo_perlin3_many(const uint8_t* P, const double* xyz, int64_t n, double* out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = uwo_perlin3(P, xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]);
+}
+
+/* perlin_util::perlin_3d_octaves (perlin_util.rs:6-22): sum_k A_k n(p f_k) / sum_k A_k with
+ * f_0 = 1, f_{k+1} = f_k * lacunarity. The reference's A_k = 0.5^k. */
+double uwo_fbm(const uint8_t* P, const uwo_params* prm, double x, double y, double z) {
+    double total = 0.0, freq = 1.0, max_value = 0.0;
+    for (int k = 0; k < prm->octaves; ++k) {
+        double amp = prm->amplitudes[k];
+        total += uwo_perlin3(P, x * freq, y * freq, z * freq) * amp;
+        max_value += amp;
+        freq *= prm->lacunarity;
+    }
+    return total / max_value;
+}
+
+/* perlin_util::iso_at (perlin_util.rs:24-29), reference-faithful f32 evaluation order:
+ *   p = (f32) fbm;  adj = ((f32) z * A) / B;  rho = (adj + p) - fmodf(adj, C)     (terrace)
+ *   p = (f32) fbm;  adj = (f32) z * slope;    rho = (adj + p) + offset            (linear)   */
+float uwo_iso_at(const uint8_t* P, const uwo_params* prm, double x, double y, double z) {
+    float p = (float)uwo_fbm(P, prm, x, y, z);
+    float z32 = (float)z;
+    if (prm->bias_kind == 0) {
+        float adj = (z32 * prm->bias_a) / prm->bias_b;
+        return (adj + p) - fmodf(adj, prm->bias_c);
+    }
+    float adj = z32 * prm->bias_a;
+    return (adj + p) + prm->bias_b;
+}
+
+/* FP64 variant of iso_at: same formula carried in double. */
+double uwo_iso_at_fp64(const uint8_t* P, const uwo_params* prm, double x, double y, double z) {
+    double p = uwo_fbm(P, prm, x, y, z);
+    if (prm->bias_kind == 0) {
+        double adj = (z * (double)prm->bias_a) / (double)prm->bias_b;
+        return (adj + p) - fmod(adj, (double)prm->bias_c);
+    }
+    return (z * (double)prm->bias_a + p) + (double)prm->bias_b;
+}
+
+/* Chunk::build_iso coordinate (chunk.rs:107-116) generalised to n cells per axis:
+ *   step = (f32) extent / (f32) n;    p = ((f64) i * (f64) step + (f64)(extent * c)) * freq
+ * For the reference, extent = 16, n = 12, freq = 1/16; multiplying by a power of two is
+ * exact, so this equals the reference's division by CHUNK_SIZE bit for bit. */
+float uwo_step(const uwo_params* prm, int32_t n) { return (float)prm->chunk_extent / (float)n; }
+
+double uwo_coord(const uwo_params* prm, int32_t c, int32_t n, int32_t i) {
+    float step = uwo_step(prm, n);
+    double local = (double)i * (double)step;
+    return (local + (double)(prm->chunk_extent * c)) * prm->frequency;
+}
+
+/* Density grid of one chunk, (n+1)^3 samples, index x (n+1)^2 + y (n+1) + z. */
+void uwo_density_ref(const uwo_params* prm, const int32_t* coord, int32_t n, float* out) {
+    uint8_t P[256];
+    uwo_perm(prm->seed, P);
+    int32_t L = n + 1;
+    for (int32_t x = 0; x < L; ++x) {
+        double px = uwo_coord(prm, coord[0], n, x);
+        for (int32_t y = 0; y < L; ++y) {
+            double py = uwo_coord(prm, coord[1], n, y);
+            for (int32_t z = 0; z < L; ++z) {
+                double pz = uwo_coord(prm, coord[2], n, z);
+                out[((int64_t)x * L + y) * L + z] = uwo_iso_at(P, prm, px, py, pz);
+            }
+        }
+    }
+}
+
+void uwo_density_fp64(const uwo_params* prm, const int32_t* coord, int32_t n, double* out) {
+    uint8_t P[256];
+    uwo_perm(prm->seed, P);
+    int32_t L = n + 1;
+    for (int32_t x = 0; x < L; ++x) {
+        double px = uwo_coord(prm, coord[0], n, x);
+        for (int32_t y = 0; y < L; ++y) {
+            double py = uwo_coord(prm, coord[1], n, y);
+            for (int32_t z = 0; z < L; ++z) {
+                double pz = uwo_coord(prm, coord[2], n, z);
+                out[((int64_t)x * L + y) * L + z] = uwo_iso_at_fp64(P, prm, px, py, pz);
+            }
+        }
+    }
+}
+
+/* Chunk::early_blank_check (chunk.rs:131-133): blank iff every sample is above iso. */
+int uwo_early_blank(const float* dens, int64_t count, float iso) {
+    for (int64_t i = 0; i < count; ++i)
+        if (!(dens[i] > iso)) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Chunk::build_mesh (chunk.rs:135-264)                                                    */
+
+typedef struct {
+    uint64_t* keys; /* (a_lin << 32) | b_lin, + 1 so that 0 means empty */
+    uint32_t* vals;
+    uint64_t mask;
+} uwo_map;
+
+static int uwo_map_init(uwo_map* m, uint64_t min_cap) {
+    uint64_t cap = 64;
+    while (cap < 2 * min_cap) cap <<= 1;
+    m->keys = (uint64_t*)calloc(cap, sizeof(uint64_t));
+    m->vals = (uint32_t*)malloc(cap * sizeof(uint32_t));
+    m->mask = cap - 1;
+    return m->keys && m->vals;
+}
+static void uwo_map_free(uwo_map* m) { free(m->keys); free(m->vals); }
+static uint32_t* uwo_map_slot(uwo_map* m, uint64_t key, int* found) {
+    uint64_t k = key + 1, h = (k * 0x9E3779B97F4A7C15ull) >> 17;
+    for (;;) {
+        h &= m->mask;
+        if (m->keys[h] == k) { *found = 1; return &m->vals[h]; }
+        if (m->keys[h] == 0) { m->keys[h] = k; *found = 0; return &m->vals[h]; }
+        ++h;
+    }
+}
+
+typedef struct {
+    int32_t n_verts, n_inds, n_active, overflow;
+} uwo_mesh_counts;
+
+/*
+ * Reference-faithful mesher. dedup_mode 0 = the reference's literal linear search over the
+ * ordered corner pairs seen so far (chunk.rs:233-243, O(V*I)); 1 = the same first-seen
+ * numbering through a hash map (identical output, used for large grids).
+ * verts: 3 floats per vertex (world position, chunk.rs:224-229); inds: u32 (the reference
+ * stores `ind as u16`, which equals this value whenever V <= 65535).
+ * cases (optional): n^3 u8 cube-case indices in scan order. tri_cells (optional): for each
+ * triangle, the linear index of the cell that emitted it (collision-triangle key, chunk.rs:173).
+ * Returns 0, or -2 if a capacity was too small (counts still hold the required sizes).
+ */
+int uwo_mesh_ref(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
+                 int dedup_mode, float* verts, int64_t vcap, uint32_t* inds, int64_t icap,
+                 uint8_t* cases, uint32_t* tri_cells, uwo_mesh_counts* counts) {
+    const int32_t L = n + 1;
+    const float scale = uwo_step(prm, n);
+    const float iso = prm->iso;
+    const float off[3] = {(float)(prm->chunk_extent * coord[0]), (float)(prm->chunk_extent * coord[1]),
+                          (float)(prm->chunk_extent * coord[2])};
+    int64_t nv = 0, ni = 0, nact = 0;
+    /* vert_pairs as linear corner indices a*S + b (chunk.rs:37,233-238) */
+    uint64_t* pairs = NULL;
+    int64_t pairs_cap = 0;
+    uwo_map map = {0};
+    if (dedup_mode == 1 && !uwo_map_init(&map, (uint64_t)5 * n * L * L + 16)) return -1;
+    const uint64_t S = (uint64_t)L * L * L;
+
+    for (int32_t x = 0; x < n; ++x)
+        for (int32_t y = 0; y < n; ++y)
+            for (int32_t z = 0; z < n; ++z) {
+                int32_t cc[8][3];
+                for (int i = 0; i < 8; ++i) {
+                    uint8_t o = UW_MC_CORNER_OFS[i];
+                    cc[i][0] = x + (o & 1);
+                    cc[i][1] = y + ((o >> 1) & 1);
+                    cc[i][2] = z + ((o >> 2) & 1);
+                }
+                uint32_t cs = 0;
+                for (int i = 0; i < 8; ++i) {
+                    float r = dens[((int64_t)cc[i][0] * L + cc[i][1]) * L + cc[i][2]];
+                    if (r < iso) cs |= 1u << i;
+                }
+                int64_t cell = ((int64_t)x * n + y) * n + z;
+                if (cases) cases[cell] = (uint8_t)cs;
+                if (cs != 0 && cs != 255) ++nact;
+                uint64_t row = UW_MC_TRI_PACKED[cs];
+                int ntri = (int)(row & 15u);
+                for (int j = 0; j < 3 * ntri; ++j) {
+                    int e = (int)((row >> (4 + 4 * j)) & 15u);
+                    int ia = UW_MC_EDGE_A_IDX[e], ib = UW_MC_EDGE_B_IDX[e];
+                    const int32_t* ca = cc[ia];
+                    const int32_t* cb = cc[ib];
+                    float sa[3], sb[3];
+                    for (int k = 0; k < 3; ++k) {
+                        sa[k] = (float)ca[k] * scale;
+                        sb[k] = (float)cb[k] * scale;
+                    }
+                    float ra = dens[((int64_t)ca[0] * L + ca[1]) * L + ca[2]];
+                    float rb = dens[((int64_t)cb[0] * L + cb[1]) * L + cb[2]];
+                    float t = (iso - ra) / (rb - ra);
+                    float m[3];
+                    for (int k = 0; k < 3; ++k) {
+                        float d = sb[k] - sa[k];
+                        m[k] = sa[k] + t * d;
+                    }
+                    uint64_t a_lin = ((uint64_t)ca[0] * L + ca[1]) * L + ca[2];
+                    uint64_t b_lin = ((uint64_t)cb[0] * L + cb[1]) * L + cb[2];
+                    uint64_t key = a_lin * S + b_lin;
+                    int64_t id = -1;
+                    if (dedup_mode == 0) {
+                        for (int64_t q = 0; q < nv; ++q)
+                            if (pairs[q] == key) { id = q; break; }
+                    } else {
+                        int found;
+                        uint32_t* slot = uwo_map_slot(&map, key, &found);
+                        if (found) id = *slot; else *slot = (uint32_t)nv;
+                    }
+                    if (id < 0) {
+                        id = nv;
+                        if (dedup_mode == 0) {
+                            if (nv == pairs_cap) {
+                                pairs_cap = pairs_cap ? 2 * pairs_cap : 1024;
+                                pairs = (uint64_t*)realloc(pairs, (size_t)pairs_cap * sizeof(uint64_t));
+                            }
+                            pairs[nv] = key;
+                        }
+                        if (nv < vcap && verts) {
+                            verts[3 * nv + 0] = m[0] + off[0];
+                            verts[3 * nv + 1] = m[1] + off[1];
+                            verts[3 * nv + 2] = m[2] + off[2];
+                        }
+                        ++nv;
+                    }
+                    if (ni < icap && inds) inds[ni] = (uint32_t)id;
+                    if (tri_cells && j % 3 == 0 && ni / 3 < icap / 3) tri_cells[ni / 3] = (uint32_t)cell;
+                    ++ni;
+                }
+            }
+    free(pairs);
+    if (dedup_mode == 1) uwo_map_free(&map);
+    counts->n_verts = (int32_t)nv;
+    counts->n_inds = (int32_t)ni;
+    counts->n_active = (int32_t)nact;
+    counts->overflow = (nv > vcap || ni > icap);
+    return counts->overflow ? -2 : 0;
+}
+
+/* FP64 mesher: same topology rule (strict < iso on the double densities), lerp and
+ * positions in double, with the same sample positions (corner * (f64)(f32)step + offset). */
+int uwo_mesh_fp64(const uwo_params* prm, const int32_t* coord, int32_t n, const double* dens,
+                  double* verts, int64_t vcap, uint32_t* inds, int64_t icap, uint8_t* cases,
+                  uwo_mesh_counts* counts) {
+    const int32_t L = n + 1;
+    const double scale = (double)uwo_step(prm, n);
+    const double iso = (double)prm->iso;
+    const double off[3] = {(double)(prm->chunk_extent * coord[0]), (double)(prm->chunk_extent * coord[1]),
+                           (double)(prm->chunk_extent * coord[2])};
+    int64_t nv = 0, ni = 0, nact = 0;
+    uwo_map map;
+    if (!uwo_map_init(&map, (uint64_t)5 * n * L * L + 16)) return -1;
+    const uint64_t S = (uint64_t)L * L * L;
+    for (int32_t x = 0; x < n; ++x)
+        for (int32_t y = 0; y < n; ++y)
+            for (int32_t z = 0; z < n; ++z) {
+                int32_t cc[8][3];
+                for (int i = 0; i < 8; ++i) {
+                    uint8_t o = UW_MC_CORNER_OFS[i];
+                    cc[i][0] = x + (o & 1);
+                    cc[i][1] = y + ((o >> 1) & 1);
+                    cc[i][2] = z + ((o >> 2) & 1);
+                }
+                uint32_t cs = 0;
+                for (int i = 0; i < 8; ++i)
+                    if (dens[((int64_t)cc[i][0] * L + cc[i][1]) * L + cc[i][2]] < iso) cs |= 1u << i;
+                int64_t cell = ((int64_t)x * n + y) * n + z;
+                if (cases) cases[cell] = (uint8_t)cs;
+                if (cs != 0 && cs != 255) ++nact;
+                uint64_t row = UW_MC_TRI_PACKED[cs];
+                int ntri = (int)(row & 15u);
+                for (int j = 0; j < 3 * ntri; ++j) {
+                    int e = (int)((row >> (4 + 4 * j)) & 15u);
+                    const int32_t* ca = cc[UW_MC_EDGE_A_IDX[e]];
+                    const int32_t* cb = cc[UW_MC_EDGE_B_IDX[e]];
+                    double ra = dens[((int64_t)ca[0] * L + ca[1]) * L + ca[2]];
+                    double rb = dens[((int64_t)cb[0] * L + cb[1]) * L + cb[2]];
+                    double t = (iso - ra) / (rb - ra);
+                    uint64_t key = ((((uint64_t)ca[0] * L + ca[1]) * L + ca[2]) * S) +
+                                   (((uint64_t)cb[0] * L + cb[1]) * L + cb[2]);
+                    int found;
+                    uint32_t* slot = uwo_map_slot(&map, key, &found);
+                    int64_t id;
+                    if (found) id = *slot;
+                    else {
+                        id = nv;
+                        *slot = (uint32_t)nv;
+                        if (nv < vcap && verts)
+                            for (int k = 0; k < 3; ++k) {
+                                double sa = (double)ca[k] * scale, sb = (double)cb[k] * scale;
+                                verts[3 * nv + k] = sa + t * (sb - sa) + off[k];
+                            }
+                        ++nv;
+                    }
+                    if (ni < icap && inds) inds[ni] = (uint32_t)id;
+                    ++ni;
+                }
+            }
+    uwo_map_free(&map);
+    counts->n_verts = (int32_t)nv;
+    counts->n_inds = (int32_t)ni;
+    counts->n_active = (int32_t)nact;
+    counts->overflow = (nv > vcap || ni > icap);
+    return counts->overflow ? -2 : 0;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Whole-chunk builds, for the CPU baseline and for bulk parity: density -> blank check ->
+ * mesh, per chunk, optionally across threads (chunk-parallel). The mesh itself is not kept
+ * beyond per-chunk counts and an order-independent checksum of the positions and indices. */
+
+typedef struct {
+    int32_t n_verts, n_inds, n_active, early_blank;
+    uint64_t checksum;
+} uwo_chunk_result;
+
+typedef struct {
+    const uwo_params* prm;
+    const int32_t* coords;
+    int32_t n;
+    int dedup_mode;
+    int64_t begin, end;
+    uwo_chunk_result* res;
+} uwo_job;
+
+static uint64_t uwo_mix(uint64_t h, uint64_t v) {
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    return h;
+}
+
+static void* uwo_worker(void* arg) {
+    uwo_job* jb = (uwo_job*)arg;
+    int32_t n = jb->n, L = n + 1;
+    int64_t S = (int64_t)L * L * L;
+    int64_t vcap = 5LL * n * L * L + 16, icap = 15LL * n * n * n + 16;
+    float* dens = (float*)malloc((size_t)S * sizeof(float));
+    float* verts = (float*)malloc((size_t)vcap * 3 * sizeof(float));
+    uint32_t* inds = (uint32_t*)malloc((size_t)icap * sizeof(uint32_t));
+    for (int64_t c = jb->begin; c < jb->end; ++c) {
+        const int32_t* co = jb->coords + 3 * c;
+        uwo_chunk_result* r = &jb->res[c];
+        uwo_density_ref(jb->prm, co, n, dens);
+        memset(r, 0, sizeof(*r));
+        if (uwo_early_blank(dens, S, jb->prm->iso)) { r->early_blank = 1; continue; }
+        uwo_mesh_counts mc;
+        uwo_mesh_ref(jb->prm, co, n, dens, jb->dedup_mode, verts, vcap, inds, icap, NULL, NULL, &mc);
+        r->n_verts = mc.n_verts;
+        r->n_inds = mc.n_inds;
+        r->n_active = mc.n_active;
+        uint64_t h = 0;
+        for (int64_t i = 0; i < mc.n_inds; ++i) {
+            uint32_t bits[3];
+            memcpy(bits, &verts[3 * inds[i]], sizeof(bits));
+            h = uwo_mix(h, ((uint64_t)i << 32) ^ bits[0] ^ ((uint64_t)bits[1] << 11) ^ ((uint64_t)bits[2] << 23));
+        }
+        r->checksum = h;
+    }
+    free(dens);
+    free(verts);
+    free(inds);
+    return NULL;
+}
+
+int uwo_build_chunks(const uwo_params* prm, const int32_t* coords, int64_t nchunks, int32_t n,
+                     int dedup_mode, int nthreads, uwo_chunk_result* res) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 256) nthreads = 256;
+    pthread_t th[256];
+    uwo_job jobs[256];
+    int64_t per = (nchunks + nthreads - 1) / nthreads;
+    int started = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        int64_t b = t * per, e = b + per;
+        if (e > nchunks) e = nchunks;
+        if (b >= e) break;
+        jobs[t] = (uwo_job){prm, coords, n, dedup_mode, b, e, res};
+        if (nthreads == 1) { uwo_worker(&jobs[t]); started = 0; break; }
+        pthread_create(&th[t], NULL, uwo_worker, &jobs[t]);
+        ++started;
+    }
+    for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
+    return 0;
+}

@@ -1,0 +1,125 @@
+"""ctypes binding of the engine's C ABI (include/uwmesh.h) -> underwaterworld_amd/libuwmesh.so.
+
+The library is loaded from the package directory only; there is no other implementation to fall
+back on, so a missing library is an error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libuwmesh.so")
+
+MAX_OCTAVES = 16
+MAX_RESOLUTION = 64
+
+UW_OK = 0
+UW_ERR_INVALID_ARG = -1
+UW_ERR_OVERFLOW = -2
+UW_ERR_HIP = -3
+UW_ERR_NO_DEVICE = -4
+UW_ERR_OOM = -5
+
+CHUNK_BLANK = 1
+CHUNK_EARLY_SKIP = 2
+CHUNK_OVERFLOW = 4
+
+P = ctypes.POINTER
+c_void_p = ctypes.c_void_p
+
+
+class UwParams(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint32),
+        ("octaves", ctypes.c_int32),
+        ("frequency", ctypes.c_double),
+        ("lacunarity", ctypes.c_double),
+        ("amplitudes", ctypes.c_double * MAX_OCTAVES),
+        ("iso", ctypes.c_float),
+        ("bias_kind", ctypes.c_int32),
+        ("bias_a", ctypes.c_float),
+        ("bias_b", ctypes.c_float),
+        ("bias_c", ctypes.c_float),
+        ("chunk_extent", ctypes.c_int32),
+    ]
+
+
+class UwChunkInfo(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint32), ("n_verts", ctypes.c_uint32), ("n_inds", ctypes.c_uint32),
+                ("n_active", ctypes.c_uint32), ("vert_off", ctypes.c_uint64), ("ind_off", ctypes.c_uint64)]
+
+
+class UwTotals(ctypes.Structure):
+    _fields_ = [("n_verts", ctypes.c_uint64), ("n_inds", ctypes.c_uint64), ("n_nonblank", ctypes.c_uint32),
+                ("overflow", ctypes.c_uint32)]
+
+
+class UwMeshOut(ctypes.Structure):
+    _fields_ = [("verts", c_void_p), ("inds", c_void_p), ("info", c_void_p), ("totals", c_void_p),
+                ("vert_cap", ctypes.c_uint64), ("ind_cap", ctypes.c_uint64)]
+
+
+# name -> (restype, argtypes); every symbol include/uwmesh.h declares.
+SIGNATURES = {
+    "uw_params_reference": (None, [P(UwParams), ctypes.c_uint32]),
+    "uw_ctx_create": (ctypes.c_int, [P(UwParams), ctypes.c_int, P(c_void_p)]),
+    "uw_ctx_destroy": (None, [c_void_p]),
+    "uw_last_error": (ctypes.c_char_p, [c_void_p]),
+    "uw_ctx_permutation": (ctypes.c_int, [c_void_p, P(ctypes.c_uint8)]),
+    "uw_permutation_table": (None, [ctypes.c_uint32, P(ctypes.c_uint8)]),
+    "uw_max_verts": (ctypes.c_uint64, [ctypes.c_int32]),
+    "uw_max_inds": (ctypes.c_uint64, [ctypes.c_int32]),
+    "uw_sample_density": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p,
+                                         c_void_p]),
+    "uw_build_chunks_plain": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, P(UwMeshOut),
+                                             c_void_p, c_void_p]),
+    "uw_chunk_build_host": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           c_void_p, ctypes.c_uint64, c_void_p, ctypes.c_uint64, P(UwChunkInfo)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libuwmesh.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("underwaterworld_amd/libuwmesh.so is not built: run "
+                               "`python -m underwaterworld_amd._build` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def make_params(p: dict) -> UwParams:
+    s = UwParams()
+    s.seed = int(p["seed"]) & 0xFFFFFFFF
+    s.octaves = int(p["octaves"])
+    s.frequency = float(p["frequency"])
+    s.lacunarity = float(p["lacunarity"])
+    amps = list(p["amplitudes"])
+    for k in range(MAX_OCTAVES):
+        s.amplitudes[k] = float(amps[k]) if k < len(amps) else 0.0
+    s.iso = float(p["iso"])
+    s.bias_kind = int(p["bias_kind"])
+    s.bias_a, s.bias_b, s.bias_c = (float(v) for v in p["bias"])
+    s.chunk_extent = int(p["chunk_extent"])
+    return s
+
+
+class UwError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("uwmesh error %d: %s" % (code, msg))
+        self.code = code
+
+
+def check(rc, ctx=None):
+    if rc != UW_OK:
+        msg = lib().uw_last_error(ctx).decode() if ctx else ""
+        raise UwError(rc, msg)

@@ -1,0 +1,214 @@
+// Context management, parameter validation, permutation table and lookup-table derivation.
+#include <math.h>
+#include <string.h>
+
+#include <new>
+
+#include "uw_internal.h"
+
+int uw_fail(uw_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+int uw_hip_fail(uw_ctx* ctx, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return uw_fail(ctx, e == hipErrorOutOfMemory ? UW_ERR_OOM : UW_ERR_HIP, m);
+}
+
+namespace {
+
+// Marsaglia xor128 step as implemented by rand_xorshift 0.3 XorShiftRng::next_u32.
+struct XorShift128 {
+    uint32_t x, y, z, w;
+    uint32_t next() {
+        uint32_t t = x ^ (x << 11);
+        x = y;
+        y = z;
+        z = w;
+        w = w ^ (w >> 19) ^ (t ^ (t >> 8));
+        return w;
+    }
+    // rand 0.8 gen_range(0..bound) for u32: Lemire widening multiply, rejecting low words above
+    // the zone (bound << leading_zeros(bound)) - 1.
+    uint32_t below(uint32_t bound) {
+        const uint32_t zone = (bound << __builtin_clz(bound)) - 1u;
+        for (;;) {
+            const uint64_t m = uint64_t(next()) * bound;
+            if (uint32_t(m) <= zone) return uint32_t(m >> 32);
+        }
+    }
+};
+
+// noise 0.8.2 PermutationTable::new(seed): the 16 seed bytes are [1,0,0,0] followed by three
+// little-endian copies of the seed, i.e. state (1, seed, seed, seed); the identity permutation
+// of 0..=255 is then Fisher-Yates shuffled from the top (rand 0.8 SliceRandom::shuffle).
+void build_permutation(uint32_t seed, uint8_t out[256]) {
+    XorShift128 rng{1u, seed, seed, seed};
+    for (int i = 0; i < 256; ++i) out[i] = uint8_t(i);
+    for (uint32_t i = 255; i > 0; --i) {
+        const uint32_t j = rng.below(i + 1);
+        const uint8_t t = out[i];
+        out[i] = out[j];
+        out[j] = t;
+    }
+}
+
+// Gradient vectors of noise 0.8.2 perlin_3d for hash & 15; codes 12..15 repeat four of the
+// twelve edge directions.
+const int8_t kGrad[16][3] = {
+    {1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
+    {0, 1, 1},  {0, -1, 1}, {0, 1, -1}, {0, -1, -1}, {1, 1, 0},  {-1, 1, 0}, {0, -1, 1}, {0, -1, -1},
+};
+
+void build_tables(uw_tables& t, const uint8_t perm[256]) {
+    memset(&t, 0, sizeof(t));
+    for (int c = 0; c < 256; ++c) {
+        const uint64_t row = UW_MC_TRI_PACKED[c];
+        t.tri[c] = row;
+        const int n = 3 * int(row & 15u);
+        uint16_t seen = 0;
+        uint16_t before[12];
+        for (int e = 0; e < 12; ++e) before[e] = 0xffff;
+        for (int j = 0; j < n; ++j) {
+            const int e = int((row >> (4 + 4 * j)) & 15u);
+            if (!(seen & (1u << e))) {
+                before[e] = seen;
+                seen = uint16_t(seen | (1u << e));
+            }
+        }
+        for (int e = 0; e < 12; ++e) t.before[c][e] = before[e] == 0xffff ? seen : before[e];
+        t.used[c] = seen;
+    }
+    memcpy(t.perm, perm, 256);
+    for (int h = 0; h < 16; ++h)
+        t.grad[h] = make_float4(float(kGrad[h][0]), float(kGrad[h][1]), float(kGrad[h][2]), 0.0f);
+}
+
+bool params_valid(const uw_params* p, std::string* why) {
+    if (p->octaves < 1 || p->octaves > UW_MAX_OCTAVES) { *why = "octaves must be in [1, 16]"; return false; }
+    if (!(p->frequency > 0.0) || !isfinite(p->frequency)) { *why = "frequency must be positive"; return false; }
+    if (!(p->lacunarity > 0.0) || !isfinite(p->lacunarity)) { *why = "lacunarity must be positive"; return false; }
+    double s = 0.0;
+    for (int k = 0; k < p->octaves; ++k) {
+        if (!isfinite(p->amplitudes[k])) { *why = "amplitudes must be finite"; return false; }
+        s += p->amplitudes[k];
+    }
+    if (s == 0.0) { *why = "amplitudes must not sum to zero"; return false; }
+    if (p->bias_kind != UW_BIAS_TERRACE && p->bias_kind != UW_BIAS_LINEAR) { *why = "unknown bias_kind"; return false; }
+    if (p->bias_kind == UW_BIAS_TERRACE && (p->bias_b == 0.0f || p->bias_c == 0.0f)) {
+        *why = "terrace bias needs nonzero divisor and modulus";
+        return false;
+    }
+    if (p->chunk_extent < 1 || p->chunk_extent > (1 << 20)) { *why = "chunk_extent out of range"; return false; }
+    if (!isfinite(p->iso)) { *why = "iso must be finite"; return false; }
+    return true;
+}
+
+}  // namespace
+
+uw_field uw_make_field(const uw_params& p) {
+    uw_field f;
+    memset(&f, 0, sizeof(f));
+    f.frequency = p.frequency;
+    double fk = 1.0, sum = 0.0;
+    for (int k = 0; k < p.octaves; ++k) {
+        f.octave_freq[k] = fk;
+        fk *= p.lacunarity;
+        f.amp[k] = float(p.amplitudes[k]);
+        sum += p.amplitudes[k];
+    }
+    f.amp_sum = float(sum);
+    f.octaves = p.octaves;
+    f.extent = p.chunk_extent;
+    f.bias_kind = p.bias_kind;
+    f.bias_a = p.bias_a;
+    f.bias_b = p.bias_b;
+    f.bias_c = p.bias_c;
+    f.iso = p.iso;
+    return f;
+}
+
+int uw_check_resolution(uw_ctx* ctx, int32_t n) {
+    if (n < 1 || n > UW_MAX_RESOLUTION) return uw_fail(ctx, UW_ERR_INVALID_ARG, "resolution must be in [1, 64]");
+    return UW_OK;
+}
+
+extern "C" {
+
+void uw_params_reference(uw_params* p, uint32_t seed) {
+    memset(p, 0, sizeof(*p));
+    p->seed = seed;
+    p->octaves = 3;                 // chunk.rs:9
+    p->frequency = 1.0 / 16.0;      // chunk.rs:108 (/ CHUNK_SIZE)
+    p->lacunarity = 2.0;            // perlin_util.rs:18
+    p->amplitudes[0] = 1.0;         // perlin_util.rs:10,17
+    p->amplitudes[1] = 0.5;
+    p->amplitudes[2] = 0.25;
+    p->iso = -0.1f;                 // chunk.rs:10
+    p->bias_kind = UW_BIAS_TERRACE; // perlin_util.rs:27-28
+    p->bias_a = 16.0f;              // CHUNK_SIZE as f32
+    p->bias_b = 32.0f;              // MAX_HEIGHT
+    p->bias_c = 0.25f;              // ADJ_Z_MOD
+    p->chunk_extent = 16;           // chunk.rs:5
+}
+
+int uw_ctx_create(const uw_params* p, int device, uw_ctx** out) {
+    if (!p || !out) return UW_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::string why;
+    if (!params_valid(p, &why)) return UW_ERR_INVALID_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return UW_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return UW_ERR_INVALID_ARG;
+    uw_ctx* ctx = new (std::nothrow) uw_ctx();
+    if (!ctx) return UW_ERR_OOM;
+    ctx->params = *p;
+    ctx->device = device;
+    build_permutation(p->seed, ctx->perm);
+    build_tables(ctx->host_tables, ctx->perm);
+    if (hipSetDevice(device) != hipSuccess ||
+        hipMalloc(&ctx->d_tables, sizeof(uw_tables)) != hipSuccess ||
+        hipMemcpy(ctx->d_tables, &ctx->host_tables, sizeof(uw_tables), hipMemcpyHostToDevice) != hipSuccess) {
+        uw_ctx_destroy(ctx);
+        return UW_ERR_HIP;
+    }
+    *out = ctx;
+    return UW_OK;
+}
+
+void uw_ctx_destroy(uw_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipFree(ctx->d_tables);
+    uw_plain_scratch& s = ctx->plain;
+    (void)hipFree(s.dens);
+    (void)hipFree(s.below);
+    (void)hipFree(s.cases);
+    (void)hipFree(s.voff);
+    (void)hipFree(s.act);
+    (void)hipFree(s.ioff);
+    (void)hipFree(ctx->d_coord1);
+    (void)hipFree(ctx->d_verts1);
+    (void)hipFree(ctx->d_inds1);
+    (void)hipFree(ctx->d_info1);
+    (void)hipFree(ctx->d_totals1);
+    delete ctx;
+}
+
+const char* uw_last_error(const uw_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int uw_ctx_permutation(const uw_ctx* ctx, uint8_t* out256) {
+    if (!ctx || !out256) return UW_ERR_INVALID_ARG;
+    memcpy(out256, ctx->perm, 256);
+    return UW_OK;
+}
+
+void uw_permutation_table(uint32_t seed, uint8_t* out256) {
+    if (out256) build_permutation(seed, out256);
+}
+
+uint64_t uw_max_verts(int32_t n) { return n < 1 ? 0 : 5ull * uint64_t(n) * uint64_t(n + 1) * uint64_t(n + 1); }
+uint64_t uw_max_inds(int32_t n) { return n < 1 ? 0 : 15ull * uint64_t(n) * uint64_t(n) * uint64_t(n); }
+
+}  // extern "C"

@@ -1,0 +1,81 @@
+// Internal declarations shared by the engine's translation units (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/uwmesh.h"
+#include "mc_table.h"
+
+// Device-resident lookup tables, uploaded once per context.
+struct uw_tables {
+    // Packed triangulation rows (see mc_table.h).
+    uint64_t tri[256];
+    // before[c][e]: 12-bit mask of the distinct edges whose first appearance in row c precedes
+    // the first appearance of edge e. A cell's k-th new vertex is its k-th first-seen edge, so the
+    // rank of edge e among a cell's owned edges is popcount(before[c][e] & owned).
+    uint16_t before[256][12];
+    // Edges used by row c (exactly the edges whose endpoints straddle the isovalue).
+    uint16_t used[256];
+    // Perlin permutation (noise 0.8.2 PermutationTable).
+    uint8_t perm[256];
+    // Gradient vector for hash code h & 15 (noise 0.8.2 perlin_3d gradient_dot_v), components in
+    // {-1, 0, 1}; w unused.
+    float4 grad[16];
+};
+
+// Scratch arrays of the plain path, grown on demand.
+struct uw_plain_scratch {
+    size_t chunks = 0;   // capacity in chunks
+    int32_t n = 0;       // resolution the arrays were sized for
+    float* dens = nullptr;
+    uint32_t* below = nullptr;
+    uint8_t* cases = nullptr;
+    uint32_t* voff = nullptr;
+    uint32_t* act = nullptr;   // (cell index) | (case << 24), compacted per chunk
+    uint32_t* ioff = nullptr;  // index offset of each compacted active cell
+};
+
+struct uw_ctx {
+    uw_params params;
+    int device = 0;
+    uint8_t perm[256];
+    uw_tables host_tables;
+    uw_tables* d_tables = nullptr;
+    uw_plain_scratch plain;
+    // single-chunk host facade buffers
+    int32_t* d_coord1 = nullptr;
+    float* d_verts1 = nullptr;
+    uint32_t* d_inds1 = nullptr;
+    uw_chunk_info* d_info1 = nullptr;
+    uw_totals* d_totals1 = nullptr;
+    uint64_t verts1_cap = 0, inds1_cap = 0;
+    std::string last_error;
+};
+
+int uw_fail(uw_ctx* ctx, int code, const std::string& msg);
+int uw_hip_fail(uw_ctx* ctx, hipError_t e, const char* what);
+
+#define UW_HIP(ctx, expr)                                        \
+    do {                                                         \
+        hipError_t _e = (expr);                                  \
+        if (_e != hipSuccess) return uw_hip_fail((ctx), _e, #expr); \
+    } while (0)
+
+// Noise-side constants derived from uw_params, passed to kernels by value.
+struct uw_field {
+    double frequency;
+    double octave_freq[UW_MAX_OCTAVES];  // lacunarity^k by repeated multiplication
+    float amp[UW_MAX_OCTAVES];           // amplitudes rounded to f32
+    float amp_sum;                       // sum of amplitudes (f64 sum rounded to f32)
+    int32_t octaves;
+    int32_t extent;
+    int32_t bias_kind;
+    float bias_a, bias_b, bias_c;
+    float iso;
+};
+
+uw_field uw_make_field(const uw_params& p);
+int uw_check_resolution(uw_ctx* ctx, int32_t n);

@@ -1,0 +1,89 @@
+// Device-side marching-cubes helpers: cube cases, edge ownership and the reference's vertex order.
+//
+// Restates Chunk::build_mesh (underwater_world/src/chunk.rs:135-264):
+//   - cells are scanned x -> y -> z (linear index (x n + y) n + z), corners in the order of
+//     chunk.rs:144-153, bit i of the case set iff rho(corner i) < iso (strict, chunk.rs:159);
+//   - each triangle-table entry names an edge (a -> b as listed in marching_table.rs:1-14),
+//     lerped at t = (iso - rho_a) / (rho_b - rho_a) in f32 (chunk.rs:178-213);
+//   - vertices are deduplicated by the ORDERED corner pair (a, b) and numbered in first-seen
+//     order (chunk.rs:233-243).
+//
+// Edge ownership. A directed edge record (a, b) is shared by two cells (x and z edges, which
+// differ in y) or four cells (y edges, which differ in x and z). The tables emit every edge whose
+// endpoints straddle the isovalue, so every sharer of an active record emits it, and the record
+// is first seen in the sharer that comes first in scan order: its OWNER. For cell (x, y, z):
+//   e4 e5 e6 e7 e10      always owned
+//   e0 e1 e2 e3          owned iff y == 0   (else by (x, y-1, z) as e4 e5 e6 e7)
+//   e8                   owned iff x == 0 and z == 0
+//   e9                   owned iff z == 0   (else by (x, y, z-1) as e10)
+//   e11                  owned iff x == 0   (else by (x-1, y, z) as e10)
+// Hence the reference's vertex id of a record is
+//   (owned active edges of all cells before the owner) + (rank of the edge among the owner's
+//   owned edges in first-appearance order),
+// which every cell can compute locally from a per-cell prefix sum.
+#pragma once
+
+#include "uw_internal.h"
+
+// Corner offsets bit0 = dx, bit1 = dy, bit2 = dz (chunk.rs:144-153).
+#define UW_CORNER_OFS_PACKED 0x67324510u /* nibbles: corner 0..7 -> 0,1,5,4,2,3,7,6 */
+#define UW_EDGE_A_PACKED 0x321076543210ull /* nibbles: edge 0..11 -> start corner */
+#define UW_EDGE_B_PACKED 0x765447650321ull /* nibbles: edge 0..11 -> end corner   */
+
+__device__ __forceinline__ uint32_t uw_corner_ofs(uint32_t i) { return (UW_CORNER_OFS_PACKED >> (4 * i)) & 15u; }
+__device__ __forceinline__ uint32_t uw_edge_a(uint32_t e) { return uint32_t(UW_EDGE_A_PACKED >> (4 * e)) & 15u; }
+__device__ __forceinline__ uint32_t uw_edge_b(uint32_t e) { return uint32_t(UW_EDGE_B_PACKED >> (4 * e)) & 15u; }
+
+__device__ __forceinline__ uint32_t uw_owned_mask(int32_t x, int32_t y, int32_t z) {
+    uint32_t m = 0x4F0u;  // e4 e5 e6 e7 e10
+    if (y == 0) m |= 0x00Fu;
+    if (x == 0 && z == 0) m |= 1u << 8;
+    if (z == 0) m |= 1u << 9;
+    if (x == 0) m |= 1u << 11;
+    return m;
+}
+
+// Owner of the record that cell (x, y, z) sees as edge e, when the cell does not own it.
+__device__ __forceinline__ void uw_edge_owner(int32_t x, int32_t y, int32_t z, uint32_t e, int32_t* ox,
+                                              int32_t* oy, int32_t* oz, uint32_t* oe) {
+    *ox = x;
+    *oy = y;
+    *oz = z;
+    if (e < 4) {
+        *oy = y - 1;
+        *oe = e + 4;
+    } else if (e == 8) {
+        if (x > 0 && z > 0) { *ox = x - 1; *oz = z - 1; *oe = 10; }
+        else if (x > 0) { *ox = x - 1; *oe = 9; }
+        else { *oz = z - 1; *oe = 11; }
+    } else if (e == 9) {
+        *oz = z - 1;
+        *oe = 10;
+    } else {  // e == 11
+        *ox = x - 1;
+        *oe = 10;
+    }
+}
+
+__device__ __forceinline__ uint32_t uw_case_of(const float r[8], float iso) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) c |= (r[i] < iso ? 1u : 0u) << i;
+    return c;
+}
+
+// World position of the lerp on edge e of cell (x, y, z) (chunk.rs:185-229), f32 with the
+// reference's operation order (the file is compiled with -ffp-contract=off).
+__device__ __forceinline__ void uw_edge_vertex(uint32_t e, int32_t x, int32_t y, int32_t z, const float r[8],
+                                               float iso, float step, float ox, float oy, float oz, float out[3]) {
+    const uint32_t ia = uw_edge_a(e), ib = uw_edge_b(e);
+    const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib);
+    const float ra = r[ia], rb = r[ib];
+    const float t = (iso - ra) / (rb - ra);
+    const float sax = float(x + int32_t(ca & 1u)) * step, sbx = float(x + int32_t(cb & 1u)) * step;
+    const float say = float(y + int32_t((ca >> 1) & 1u)) * step, sby = float(y + int32_t((cb >> 1) & 1u)) * step;
+    const float saz = float(z + int32_t((ca >> 2) & 1u)) * step, sbz = float(z + int32_t((cb >> 2) & 1u)) * step;
+    out[0] = (sax + t * (sbx - sax)) + ox;
+    out[1] = (say + t * (sby - say)) + oy;
+    out[2] = (saz + t * (sbz - saz)) + oz;
+}

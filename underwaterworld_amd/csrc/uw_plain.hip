@@ -1,0 +1,439 @@
+// Plain path: each stage of the chunk builder is one kernel launch over every chunk.
+//
+//   k_noise_grid   density of every sample (Chunk::build_iso, chunk.rs:105-129) + "some sample is
+//                  not above iso" flag (the negation of early_blank_check, chunk.rs:131-133)
+//   k_classify     cube case of every cell, per-cell vertex/index counts, chunk-wide prefix sums
+//                  and the compacted list of active cells (chunk.rs:141-164)
+//   k_offsets      exclusive scan of per-chunk sizes into output offsets, capacity check
+//   k_emit         edge lerps of owned records and the index buffer (chunk.rs:164-243)
+//
+// HBM layout (scratch, per chunk c of resolution n, S = (n+1)^3 samples, C = n^3 cells):
+//   dens  [c][S] f32      x-major, z fastest (chunk.rs:351-353)
+//   cases [c][C] u8       scan order (x n + y) n + z
+//   voff  [c][C] u32      vertex id of the cell's first owned record (chunk-local)
+//   act   [c][C] u32      compacted active cells: cell | case << 24
+//   ioff  [c][C] u32      index offset of each compacted active cell
+#include "uw_mesh.h"
+#include "uw_noise.h"
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Density grid. One workgroup per (chunk, slab of `planes` x-planes). Per-axis octave data for the
+// slab's x range and the full y, z ranges are built once in LDS, then every sample combines them.
+__global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_tables* __restrict__ tab,
+                                                         const int32_t* __restrict__ coords, int32_t n,
+                                                         float step, int32_t planes, int32_t slabs,
+                                                         float* __restrict__ dens, uint32_t* __restrict__ below) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int32_t L = n + 1, O = f.octaves;
+    const int32_t chunk = blockIdx.x / slabs, slab = blockIdx.x % slabs;
+    const int32_t x0 = slab * planes;
+    const int32_t nx = min(planes, L - x0);
+    float4* GP = reinterpret_cast<float4*>(smem);                  // 256 x 16 B
+    uint8_t* P = smem + 256 * sizeof(float4);                     // 256 B
+    uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);             // [O][planes]
+    uw_axis* AY = AX + O * planes;                                 // [O][L]
+    uw_axis* AZ = AY + O * L;                                      // [O][L]
+    float* ADJ = reinterpret_cast<float*>(AZ + O * L);             // [L]
+    float* POST = ADJ + L;                                         // [L]
+
+    const int32_t cx = coords[3 * chunk], cy = coords[3 * chunk + 1], cz = coords[3 * chunk + 2];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+        const uint8_t p = tab->perm[i];
+        P[i] = p;
+        GP[i] = tab->grad[p & 15];
+    }
+    const int32_t nent = O * (nx + 2 * L);
+    for (int e = threadIdx.x; e < nent; e += blockDim.x) {
+        const int32_t k = e % O;
+        const int32_t r = e / O;
+        const double fk = f.octave_freq[k];
+        if (r < nx) {
+            AX[k * planes + r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, cx, x0 + r, step), fk), tab->perm);
+        } else if (r < nx + L) {
+            const int32_t i = r - nx;
+            AY[k * L + i] = uw_axis_of(uw_sample_coord(f, cy, i, step), fk);
+        } else {
+            const int32_t i = r - nx - L;
+            AZ[k * L + i] = uw_axis_of(uw_sample_coord(f, cz, i, step), fk);
+        }
+    }
+    for (int i = threadIdx.x; i < L; i += blockDim.x) uw_bias_terms(f, uw_sample_coord(f, cz, i, step), &ADJ[i], &POST[i]);
+    __syncthreads();
+
+    const int32_t LL = L * L;
+    const int32_t count = nx * LL;
+    const uint64_t base = uint64_t(chunk) * uint64_t(LL) * uint64_t(L) + uint64_t(x0) * uint64_t(LL);
+    int any_below = 0;
+    for (int32_t s = threadIdx.x; s < count; s += blockDim.x) {
+        const int32_t ix = s / LL;
+        const int32_t rem = s - ix * LL;
+        const int32_t iy = rem / L;
+        const int32_t iz = rem - iy * L;
+        float total = 0.0f;
+        for (int32_t k = 0; k < O; ++k) {
+            const float v = uw_perlin_octave(AX[k * planes + ix], AY[k * L + iy], AZ[k * L + iz], P, GP);
+            total = fmaf(f.amp[k], v, total);
+        }
+        const float rho = uw_density_from(total, f.amp_sum, ADJ[iz], POST[iz]);
+        dens[base + s] = rho;
+        any_below |= !(rho > f.iso);
+    }
+    any_below = __syncthreads_or(any_below);
+    if (threadIdx.x == 0 && any_below) atomicOr(&below[chunk], 1u);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Block-wide exclusive scan of one u32 per thread (kThreads threads, wave64).
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* lds_waves) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) lds_waves[wave] = inc;
+    __syncthreads();
+    uint32_t wave_base = 0, sum = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+        const uint32_t t = lds_waves[w];
+        if (w < wave) wave_base += t;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return wave_base + inc - v;
+}
+
+// Cube cases, counts, per-chunk prefix sums and active-cell compaction. One workgroup per chunk.
+__global__ void __launch_bounds__(kThreads) k_classify(float iso, const uw_tables* __restrict__ tab, int32_t n,
+                                                       const uint32_t* __restrict__ below,
+                                                       const float* __restrict__ dens, uint8_t* __restrict__ cases,
+                                                       uint32_t* __restrict__ voff, uint32_t* __restrict__ act,
+                                                       uint32_t* __restrict__ ioff, uw_chunk_info* __restrict__ info) {
+    __shared__ uint32_t lut[256];  // used-edge mask | triangle count << 16
+    __shared__ uint32_t waves[kThreads / 64];
+    const int32_t chunk = blockIdx.x;
+    if (!below[chunk]) {
+        if (threadIdx.x == 0) {
+            uw_chunk_info r = {UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP, 0, 0, 0, 0, 0};
+            info[chunk] = r;
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
+    __syncthreads();
+    const int32_t L = n + 1, C = n * n * n, nn = n * n;
+    const float* D = dens + uint64_t(chunk) * uint64_t(L) * uint64_t(L) * uint64_t(L);
+    const uint64_t cbase = uint64_t(chunk) * uint64_t(C);
+    uint32_t carry_v = 0, carry_i = 0, carry_a = 0;
+    for (int32_t r0 = 0; r0 < C; r0 += kThreads) {
+        const int32_t cell = r0 + threadIdx.x;
+        uint32_t packed = 0, cs = 0;
+        if (cell < C) {
+            const int32_t x = cell / nn, y = (cell / n) % n, z = cell % n;
+            float r[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t o = uw_corner_ofs(i);
+                r[i] = D[(x + int32_t(o & 1u)) * L * L + (y + int32_t((o >> 1) & 1u)) * L + z + int32_t((o >> 2) & 1u)];
+            }
+            cs = uw_case_of(r, iso);
+            const uint32_t w = lut[cs];
+            const uint32_t nv = __popc(w & 0xFFFu & uw_owned_mask(x, y, z));
+            const uint32_t nt = w >> 16;
+            const uint32_t a = (cs != 0u && cs != 255u) ? 1u : 0u;
+            packed = nv | (nt << 12) | (a << 23);
+            cases[cbase + cell] = uint8_t(cs);
+        }
+        uint32_t tot;
+        const uint32_t pre = block_exclusive_scan(packed, &tot, waves);
+        if (cell < C) {
+            voff[cbase + cell] = carry_v + (pre & 0xFFFu);
+            if (packed >> 23) {
+                const uint32_t slot = carry_a + (pre >> 23);
+                act[cbase + slot] = uint32_t(cell) | (cs << 24);
+                ioff[cbase + slot] = carry_i + 3u * ((pre >> 12) & 0x7FFu);
+            }
+        }
+        carry_v += tot & 0xFFFu;
+        carry_i += 3u * ((tot >> 12) & 0x7FFu);
+        carry_a += tot >> 23;
+    }
+    if (threadIdx.x == 0) {
+        uw_chunk_info r = {carry_i == 0 ? uint32_t(UW_CHUNK_BLANK) : 0u, carry_v, carry_i, carry_a, 0, 0};
+        info[chunk] = r;
+    }
+}
+
+// Exclusive scan of per-chunk sizes into output offsets. One workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) k_offsets(uw_chunk_info* __restrict__ info, int32_t nchunks, uint64_t vcap,
+                                                  uint64_t icap, uw_totals* __restrict__ totals) {
+    __shared__ unsigned long long wv[16], wi[16];
+    __shared__ unsigned int wn[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long carry_v = 0, carry_i = 0;
+    unsigned int carry_n = 0, ovf = 0;
+    for (int32_t c0 = 0; c0 < nchunks; c0 += 1024) {
+        const int32_t c = c0 + threadIdx.x;
+        uw_chunk_info r = {};
+        if (c < nchunks) r = info[c];
+        const unsigned long long v = r.n_verts, i = r.n_inds;
+        const unsigned int nb = (c < nchunks && r.n_inds > 0) ? 1u : 0u;
+        unsigned long long sv = v, si = i;
+        unsigned int sn = nb;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const unsigned long long tv = __shfl_up(sv, d, 64), ti = __shfl_up(si, d, 64);
+            const unsigned int tn = __shfl_up(sn, d, 64);
+            if (lane >= d) { sv += tv; si += ti; sn += tn; }
+        }
+        if (lane == 63) { wv[wave] = sv; wi[wave] = si; wn[wave] = sn; }
+        __syncthreads();
+        unsigned long long bv = 0, bi = 0, tv = 0, ti = 0;
+        unsigned int tn = 0;
+        for (int w = 0; w < 16; ++w) {
+            if (w < wave) { bv += wv[w]; bi += wi[w]; }
+            tv += wv[w];
+            ti += wi[w];
+            tn += wn[w];
+        }
+        __syncthreads();
+        if (c < nchunks) {
+            const unsigned long long vo = carry_v + bv + sv - v, io = carry_i + bi + si - i;
+            r.vert_off = vo;
+            r.ind_off = io;
+            if (vo + v > vcap || io + i > icap) {
+                r.flags |= UW_CHUNK_OVERFLOW;
+                ovf = 1;
+            }
+            info[c] = r;
+        }
+        carry_v += tv;
+        carry_i += ti;
+        carry_n += tn;
+    }
+    ovf = __syncthreads_or(ovf);
+    if (threadIdx.x == 0) {
+        uw_totals t;
+        t.n_verts = carry_v;
+        t.n_inds = carry_i;
+        t.n_nonblank = carry_n;
+        t.overflow = ovf ? 1u : 0u;
+        *totals = t;
+    }
+}
+
+// Vertex lerps of owned records and the index buffer. One workgroup per chunk, one thread per
+// active cell.
+__global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, const uw_tables* __restrict__ tab,
+                                                   const int32_t* __restrict__ coords, int32_t n, float step,
+                                                   const float* __restrict__ dens, const uint8_t* __restrict__ cases,
+                                                   const uint32_t* __restrict__ voff, const uint32_t* __restrict__ act,
+                                                   const uint32_t* __restrict__ ioff,
+                                                   const uw_chunk_info* __restrict__ info, float* __restrict__ verts,
+                                                   uint32_t* __restrict__ inds) {
+    const int32_t chunk = blockIdx.x;
+    const uw_chunk_info ci = info[chunk];
+    if (ci.flags & (UW_CHUNK_BLANK | UW_CHUNK_OVERFLOW)) return;
+    const int32_t L = n + 1, C = n * n * n, nn = n * n;
+    const float* D = dens + uint64_t(chunk) * uint64_t(L) * uint64_t(L) * uint64_t(L);
+    const uint64_t cbase = uint64_t(chunk) * uint64_t(C);
+    const float ox = float(extent * coords[3 * chunk]), oy = float(extent * coords[3 * chunk + 1]),
+                oz = float(extent * coords[3 * chunk + 2]);
+    for (uint32_t a = threadIdx.x; a < ci.n_active; a += blockDim.x) {
+        const uint32_t ent = act[cbase + a];
+        const int32_t cell = int32_t(ent & 0xFFFFFFu);
+        const uint32_t cs = ent >> 24;
+        const int32_t x = cell / nn, y = (cell / n) % n, z = cell % n;
+        float r[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const uint32_t o = uw_corner_ofs(i);
+            r[i] = D[(x + int32_t(o & 1u)) * L * L + (y + int32_t((o >> 1) & 1u)) * L + z + int32_t((o >> 2) & 1u)];
+        }
+        const uint32_t owned = uw_owned_mask(x, y, z);
+        const uint64_t row = tab->tri[cs];
+        const int nidx = 3 * int(row & 15u);
+        const uint32_t myv = voff[cbase + cell];
+        const uint64_t vb = ci.vert_off + myv;
+        const uint64_t ib = ci.ind_off + ioff[cbase + a];
+        uint32_t seen = 0;
+        for (int j = 0; j < nidx; ++j) {
+            const uint32_t e = uint32_t(row >> (4 + 4 * j)) & 15u;
+            const uint32_t bit = 1u << e;
+            uint32_t id;
+            if (owned & bit) {
+                const uint32_t rank = __popc(tab->before[cs][e] & owned);
+                id = myv + rank;
+                if (!(seen & bit)) {
+                    float p[3];
+                    uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
+                    float* dst = verts + 3 * (vb + rank);
+                    dst[0] = p[0];
+                    dst[1] = p[1];
+                    dst[2] = p[2];
+                }
+            } else {
+                int32_t qx, qy, qz;
+                uint32_t qe;
+                uw_edge_owner(x, y, z, e, &qx, &qy, &qz, &qe);
+                const int32_t q = (qx * n + qy) * n + qz;
+                const uint32_t qc = cases[cbase + q];
+                id = voff[cbase + q] + __popc(tab->before[qc][qe] & uw_owned_mask(qx, qy, qz));
+            }
+            seen |= bit;
+            inds[ib + j] = id;
+        }
+    }
+}
+
+// LDS bytes of k_noise_grid.
+size_t noise_lds_bytes(int32_t O, int32_t planes, int32_t L) {
+    return 256 * sizeof(float4) + 256 + sizeof(uw_axis) * size_t(O) * size_t(planes + 2 * L) + 2 * sizeof(float) * L;
+}
+
+int32_t noise_planes(int32_t L) {
+    int32_t p = (2048 + L * L - 1) / (L * L);
+    return p < 1 ? 1 : (p > L ? L : p);
+}
+
+int ensure_plain_scratch(uw_ctx* ctx, int32_t nchunks, int32_t n) {
+    uw_plain_scratch& s = ctx->plain;
+    const size_t L = size_t(n) + 1, S = L * L * L, C = size_t(n) * n * n;
+    const size_t need_dens = size_t(nchunks) * S, need_cells = size_t(nchunks) * C;
+    const size_t have_dens = s.chunks * (size_t(s.n) + 1) * (size_t(s.n) + 1) * (size_t(s.n) + 1);
+    const size_t have_cells = s.chunks * size_t(s.n) * s.n * s.n;
+    if (s.dens && need_dens <= have_dens && need_cells <= have_cells && size_t(nchunks) <= s.chunks) return UW_OK;
+    (void)hipFree(s.dens);
+    (void)hipFree(s.below);
+    (void)hipFree(s.cases);
+    (void)hipFree(s.voff);
+    (void)hipFree(s.act);
+    (void)hipFree(s.ioff);
+    s = uw_plain_scratch();
+    UW_HIP(ctx, hipMalloc(&s.dens, need_dens * sizeof(float)));
+    UW_HIP(ctx, hipMalloc(&s.below, size_t(nchunks) * sizeof(uint32_t)));
+    UW_HIP(ctx, hipMalloc(&s.cases, need_cells));
+    UW_HIP(ctx, hipMalloc(&s.voff, need_cells * sizeof(uint32_t)));
+    UW_HIP(ctx, hipMalloc(&s.act, need_cells * sizeof(uint32_t)));
+    UW_HIP(ctx, hipMalloc(&s.ioff, need_cells * sizeof(uint32_t)));
+    s.chunks = size_t(nchunks);
+    s.n = n;
+    return UW_OK;
+}
+
+int launch_noise(uw_ctx* ctx, const uw_field& f, const int32_t* coords, int32_t nchunks, int32_t n, float* dens,
+                 uint32_t* below, hipStream_t st) {
+    const int32_t L = n + 1;
+    const int32_t planes = noise_planes(L);
+    const int32_t slabs = (L + planes - 1) / planes;
+    const float step = float(f.extent) / float(n);
+    const size_t lds = noise_lds_bytes(f.octaves, planes, L);
+    const uint64_t blocks = uint64_t(nchunks) * uint64_t(slabs);
+    if (blocks > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
+    hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((nchunks + 255) / 256)), dim3(256), 0, st, below, uint64_t(nchunks), 0u);
+    hipLaunchKernelGGL(k_noise_grid, dim3(uint32_t(blocks)), dim3(kThreads), lds, st, f, ctx->d_tables, coords, n,
+                       step, planes, slabs, dens, below);
+    UW_HIP(ctx, hipGetLastError());
+    return UW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uw_sample_density(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, int32_t n, float* dens, uint32_t* below,
+                      void* stream) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    if (int rc = uw_check_resolution(ctx, n)) return rc;
+    if (nchunks < 0 || (nchunks > 0 && (!coords || !dens))) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    if (nchunks == 0) return UW_OK;
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    const uw_field f = uw_make_field(ctx->params);
+    uint32_t* flags = below;
+    if (!flags) {
+        if (int rc = ensure_plain_scratch(ctx, nchunks, n)) return rc;
+        flags = ctx->plain.below;
+    }
+    return launch_noise(ctx, f, coords, nchunks, n, dens, flags, static_cast<hipStream_t>(stream));
+}
+
+int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, int32_t n, const uw_mesh_out* out,
+                          uint8_t* cases, void* stream) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    if (int rc = uw_check_resolution(ctx, n)) return rc;
+    if (nchunks < 0 || !out || !out->info || !out->totals || (nchunks > 0 && !coords) ||
+        (out->vert_cap > 0 && !out->verts) || (out->ind_cap > 0 && !out->inds))
+        return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (nchunks == 0) {
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(out->totals),
+                           uint64_t(sizeof(uw_totals) / 4), 0u);
+        UW_HIP(ctx, hipGetLastError());
+        return UW_OK;
+    }
+    if (int rc = ensure_plain_scratch(ctx, nchunks, n)) return rc;
+    uw_plain_scratch& s = ctx->plain;
+    const uw_field f = uw_make_field(ctx->params);
+    if (int rc = launch_noise(ctx, f, coords, nchunks, n, s.dens, s.below, st)) return rc;
+    uint8_t* cs = cases ? cases : s.cases;
+    hipLaunchKernelGGL(k_classify, dim3(uint32_t(nchunks)), dim3(kThreads), 0, st, f.iso, ctx->d_tables, n, s.below,
+                       s.dens, cs, s.voff, s.act, s.ioff, out->info);
+    hipLaunchKernelGGL(k_offsets, dim3(1), dim3(1024), 0, st, out->info, nchunks, out->vert_cap, out->ind_cap,
+                       out->totals);
+    const float step = float(f.extent) / float(n);
+    hipLaunchKernelGGL(k_emit, dim3(uint32_t(nchunks)), dim3(kThreads), 0, st, f.iso, f.extent, ctx->d_tables, coords,
+                       n, step, s.dens, cs, s.voff, s.act, s.ioff, out->info, out->verts, out->inds);
+    UW_HIP(ctx, hipGetLastError());
+    return UW_OK;
+}
+
+int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, float* verts, uint64_t vert_cap,
+                        uint32_t* inds, uint64_t ind_cap, uw_chunk_info* info) {
+    if (!ctx || !info) return UW_ERR_INVALID_ARG;
+    if (int rc = uw_check_resolution(ctx, n)) return rc;
+    if ((vert_cap > 0 && !verts) || (ind_cap > 0 && !inds)) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    const uint64_t vmax = uw_max_verts(n), imax = uw_max_inds(n);
+    if (!ctx->d_coord1) {
+        UW_HIP(ctx, hipMalloc(&ctx->d_coord1, 3 * sizeof(int32_t)));
+        UW_HIP(ctx, hipMalloc(&ctx->d_info1, sizeof(uw_chunk_info)));
+        UW_HIP(ctx, hipMalloc(&ctx->d_totals1, sizeof(uw_totals)));
+    }
+    if (ctx->verts1_cap < vmax || ctx->inds1_cap < imax) {
+        (void)hipFree(ctx->d_verts1);
+        (void)hipFree(ctx->d_inds1);
+        ctx->d_verts1 = nullptr;
+        ctx->d_inds1 = nullptr;
+        ctx->verts1_cap = ctx->inds1_cap = 0;
+        UW_HIP(ctx, hipMalloc(&ctx->d_verts1, vmax * 3 * sizeof(float)));
+        UW_HIP(ctx, hipMalloc(&ctx->d_inds1, imax * sizeof(uint32_t)));
+        ctx->verts1_cap = vmax;
+        ctx->inds1_cap = imax;
+    }
+    const int32_t c[3] = {cx, cy, cz};
+    UW_HIP(ctx, hipMemcpy(ctx->d_coord1, c, sizeof(c), hipMemcpyHostToDevice));
+    uw_mesh_out out = {ctx->d_verts1, ctx->d_inds1, ctx->d_info1, ctx->d_totals1, ctx->verts1_cap, ctx->inds1_cap};
+    if (int rc = uw_build_chunks_plain(ctx, ctx->d_coord1, 1, n, &out, nullptr, nullptr)) return rc;
+    UW_HIP(ctx, hipMemcpy(info, ctx->d_info1, sizeof(uw_chunk_info), hipMemcpyDeviceToHost));
+    if (info->n_verts > vert_cap || info->n_inds > ind_cap)
+        return uw_fail(ctx, UW_ERR_OVERFLOW, "output capacity too small for this chunk");
+    if (info->n_verts)
+        UW_HIP(ctx, hipMemcpy(verts, ctx->d_verts1, size_t(info->n_verts) * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    if (info->n_inds)
+        UW_HIP(ctx, hipMemcpy(inds, ctx->d_inds1, size_t(info->n_inds) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return UW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,239 @@
+"""Host-side mirror of the reference chunk-builder interface, over the C ABI.
+
+    Perlin(seed, params)            noise::Perlin::new(seed)            state.rs:358-359
+    Chunk(pos)                      Chunk::new(pos)                     chunk.rs:89-103
+    Chunk.build_full(perlin)        Chunk::build_full                   chunk.rs:266-268
+    Chunk.build_partial(perlin)     Chunk::build_partial -> finished    chunk.rs:270-313
+    Chunk.not_blank()               Chunk::not_blank                    chunk.rs:344
+    Chunk.verts_buffer_slice()      Chunk::verts_buffer_slice (xyz)     chunk.rs:346
+    Chunk.inds_buffer_slice()       Chunk::inds_buffer_slice (u32)      chunk.rs:347
+    Chunk.num_inds()                Chunk::num_inds                     chunk.rs:348
+
+and the device-resident multi-chunk forms (torch tensors on the context's device):
+
+    build_chunks_plain(perlin, coords, n)   one launch per builder stage over all chunks
+    sample_density(perlin, coords, n)       iso_at over every sample of every chunk
+
+The reference's `device: &wgpu::Device` argument is the renderer's upload target; the engine's
+output buffers are HIP device memory (or host arrays for the Chunk facade), so it is accepted and
+ignored for signature parity.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from . import params as _params
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+class Perlin:
+    """Density-field context: the permutation table of `seed` plus the field parameters, uploaded
+    to one GPU (noise::Perlin::new, state.rs:358-359)."""
+
+    def __init__(self, params: dict | None = None, seed: int | None = None, device: int = 0):
+        p = dict(params) if params is not None else _params.preset("reference")
+        if seed is not None:
+            p["seed"] = int(seed)
+        self.params = p
+        self.device = device
+        self._ctx = ctypes.c_void_p()
+        s = _lib.make_params(p)
+        rc = _lib.lib().uw_ctx_create(ctypes.byref(s), device, ctypes.byref(self._ctx))
+        if rc != _lib.UW_OK:
+            raise _lib.UwError(rc, "uw_ctx_create failed (parameters or device)")
+
+    @classmethod
+    def new(cls, seed: int):
+        """noise::Perlin::new(seed) with the reference crate's constants."""
+        return cls(None, seed=seed)
+
+    @property
+    def ctx(self):
+        return self._ctx
+
+    def permutation(self) -> np.ndarray:
+        out = np.zeros(256, np.uint8)
+        _lib.check(_lib.lib().uw_ctx_permutation(self._ctx, out.ctypes.data_as(_lib.P(ctypes.c_uint8))), self._ctx)
+        return out
+
+    def close(self):
+        if self._ctx:
+            _lib.lib().uw_ctx_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def permutation_table(seed: int) -> np.ndarray:
+    """Host-only PermutationTable::new(seed)."""
+    out = np.zeros(256, np.uint8)
+    _lib.lib().uw_permutation_table(int(seed) & 0xFFFFFFFF, out.ctypes.data_as(_lib.P(ctypes.c_uint8)))
+    return out
+
+
+class Chunk:
+    """One chunk, built through the engine in a single call (chunk.rs:80-349)."""
+
+    def __init__(self, pos, resolution: int | None = None):
+        self.pos = tuple(int(v) for v in pos)
+        self.resolution = resolution
+        self._verts = None
+        self._inds = None
+        self._info = None
+        self._done = False
+
+    @classmethod
+    def new(cls, pos):
+        return cls(pos)
+
+    def build_full(self, perlin: Perlin, device=None):
+        while not self.build_partial(perlin, device):
+            pass
+
+    def build_partial(self, perlin: Perlin, device=None) -> bool:
+        """The reference slices a build over several frames (chunk.rs:19-20,270-313); the GPU
+        build completes in one call, which returns True ("finished")."""
+        if self._done:
+            return True
+        n = int(self.resolution or perlin.params.get("resolution", 12))
+        L = _lib.lib()
+        vcap, icap = int(L.uw_max_verts(n)), int(L.uw_max_inds(n))
+        verts = np.empty((vcap, 3), np.float32)
+        inds = np.empty(icap, np.uint32)
+        info = _lib.UwChunkInfo()
+        rc = L.uw_chunk_build_host(perlin.ctx, self.pos[0], self.pos[1], self.pos[2], n,
+                                   verts.ctypes.data_as(ctypes.c_void_p), vcap,
+                                   inds.ctypes.data_as(ctypes.c_void_p), icap, ctypes.byref(info))
+        _lib.check(rc, perlin.ctx)
+        self._info = info
+        if info.n_inds > 0:
+            self._verts = verts[: info.n_verts].copy()
+            self._inds = inds[: info.n_inds].copy()
+        self._done = True
+        return True
+
+    def not_blank(self) -> bool:
+        return self._verts is not None
+
+    def early_skipped(self) -> bool:
+        return bool(self._info and self._info.flags & _lib.CHUNK_EARLY_SKIP)
+
+    def verts_buffer_slice(self) -> np.ndarray:
+        # the reference's accessor panics on a blank chunk (chunk.rs:345-346)
+        if self._verts is None:
+            raise RuntimeError("verts_buffer_slice() on a blank chunk")
+        return self._verts
+
+    def inds_buffer_slice(self) -> np.ndarray:
+        if self._inds is None:
+            raise RuntimeError("inds_buffer_slice() on a blank chunk")
+        return self._inds
+
+    def num_inds(self) -> int:
+        return int(self._info.n_inds) if self._info is not None else 0
+
+    def num_active(self) -> int:
+        return int(self._info.n_active) if self._info is not None else 0
+
+
+# ---------------------------------------------------------------------------------------------
+# Device-resident multi-chunk forms.
+
+INFO_DTYPE = np.dtype([("flags", "<u4"), ("n_verts", "<u4"), ("n_inds", "<u4"), ("n_active", "<u4"),
+                       ("vert_off", "<u8"), ("ind_off", "<u8")])
+TOTALS_DTYPE = np.dtype([("n_verts", "<u8"), ("n_inds", "<u8"), ("n_nonblank", "<u4"), ("overflow", "<u4")])
+
+
+@dataclass
+class MeshBatch:
+    """Output of a multi-chunk build: device tensors plus host copies of the small records."""
+    verts: "torch.Tensor"   # (vert_cap, 3) f32
+    inds: "torch.Tensor"    # (ind_cap,) i32 (u32 bits)
+    info_raw: "torch.Tensor"  # (nchunks * 32,) u8
+    totals_raw: "torch.Tensor"  # (24,) u8
+    cases: "torch.Tensor | None"
+
+    def info(self) -> np.ndarray:
+        return self.info_raw.cpu().numpy().view(INFO_DTYPE)
+
+    def totals(self) -> np.ndarray:
+        return self.totals_raw.cpu().numpy().view(TOTALS_DTYPE)[0]
+
+    def chunk_mesh(self, i: int, info=None):
+        """(verts (V,3) f32, inds (I,) u32) of chunk i, as numpy arrays."""
+        r = (info if info is not None else self.info())[i]
+        v0, nv, i0, ni = int(r["vert_off"]), int(r["n_verts"]), int(r["ind_off"]), int(r["n_inds"])
+        v = self.verts[v0:v0 + nv].cpu().numpy()
+        ix = self.inds[i0:i0 + ni].cpu().numpy().view(np.uint32)
+        return v, ix
+
+
+def _alloc_out(nchunks: int, vert_cap: int, ind_cap: int, device):
+    import torch
+    verts = torch.empty((max(vert_cap, 1), 3), dtype=torch.float32, device=device)
+    inds = torch.empty((max(ind_cap, 1),), dtype=torch.int32, device=device)
+    info = torch.empty((max(nchunks, 1) * 32,), dtype=torch.uint8, device=device)
+    totals = torch.empty((24,), dtype=torch.uint8, device=device)
+    return verts, inds, info, totals
+
+
+def _mesh_out(verts, inds, info, totals, vert_cap, ind_cap):
+    return _lib.UwMeshOut(verts.data_ptr(), inds.data_ptr(), info.data_ptr(), totals.data_ptr(), vert_cap, ind_cap)
+
+
+def _coords_tensor(coords, device):
+    import torch
+    if isinstance(coords, torch.Tensor):
+        t = coords.to(device=device, dtype=torch.int32).contiguous().view(-1, 3)
+    else:
+        t = torch.as_tensor(np.asarray(coords, np.int32).reshape(-1, 3), device=device)
+    return t
+
+
+def build_chunks_plain(perlin: Perlin, coords, n: int, vert_cap: int | None = None, ind_cap: int | None = None,
+                       want_cases: bool = False, stream=None) -> MeshBatch:
+    import torch
+    dev = torch.device("cuda", perlin.device)
+    ct = _coords_tensor(coords, dev)
+    nch = ct.shape[0]
+    L = _lib.lib()
+    if vert_cap is None:
+        vert_cap = int(L.uw_max_verts(n)) * nch
+    if ind_cap is None:
+        ind_cap = int(L.uw_max_inds(n)) * nch
+    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev)
+    cases = torch.empty((max(nch, 1) * n ** 3,), dtype=torch.uint8, device=dev) if want_cases else None
+    out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
+    rc = L.uw_build_chunks_plain(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), nch, n, ctypes.byref(out),
+                                 ctypes.c_void_p(cases.data_ptr() if cases is not None else 0), _stream_ptr(stream))
+    _lib.check(rc, perlin.ctx)
+    return MeshBatch(verts, inds, info[: nch * 32], totals, cases)
+
+
+def sample_density(perlin: Perlin, coords, n: int, stream=None):
+    """Density grids (nchunks, (n+1)^3) f32 and the not-early-blank flags (nchunks,) as tensors."""
+    import torch
+    dev = torch.device("cuda", perlin.device)
+    ct = _coords_tensor(coords, dev)
+    nch = ct.shape[0]
+    dens = torch.empty((max(nch, 1), (n + 1) ** 3), dtype=torch.float32, device=dev)
+    below = torch.empty((max(nch, 1),), dtype=torch.int32, device=dev)
+    rc = _lib.lib().uw_sample_density(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), nch, n,
+                                      ctypes.c_void_p(dens.data_ptr()), ctypes.c_void_p(below.data_ptr()),
+                                      _stream_ptr(stream))
+    _lib.check(rc, perlin.ctx)
+    return dens[:nch], below[:nch]
