@@ -69,6 +69,7 @@ enum uw_chunk_flags {
     UW_CHUNK_BLANK = 1,      /* no triangles: the reference's !not_blank() (chunk.rs:291,344) */
     UW_CHUNK_EARLY_SKIP = 2, /* every sample above iso: early_blank_check (chunk.rs:131-133)  */
     UW_CHUNK_OVERFLOW = 4,   /* did not fit the output capacity; nothing written for it       */
+    UW_CHUNK_INVALID = 8,    /* resolution outside the accepted range; nothing built          */
 };
 
 typedef struct uw_chunk_info {
@@ -117,6 +118,13 @@ int uw_ctx_permutation(const uw_ctx* ctx, uint8_t* out256);
 /* Host-only: the permutation table noise::Perlin::new(seed) builds (no device needed). */
 void uw_permutation_table(uint32_t seed, uint8_t* out256);
 
+/* Kernel timing: while enabled, every build records HIP events on its stream around its dominant
+ * kernel (the density kernel of the plain path, the persistent kernel of the batched path).
+ * uw_profile_read synchronises on the recorded events, returns the summed milliseconds and the
+ * number of timed launches since the last read, and resets both. Not for use under graph capture. */
+int uw_profile_enable(uw_ctx* ctx, int on);
+int uw_profile_read(uw_ctx* ctx, double* total_ms, uint64_t* launches);
+
 /* Upper bounds for one chunk of n cells per axis: vertices 5 n (n+1)^2, indices 15 n^3. */
 uint64_t uw_max_verts(int32_t n);
 uint64_t uw_max_inds(int32_t n);
@@ -139,6 +147,40 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
  * UW_ERR_OVERFLOW nothing is copied and info->n_verts / n_inds give the sizes needed. */
 int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, float* verts,
                         uint64_t vert_cap, uint32_t* inds, uint64_t ind_cap, uw_chunk_info* info);
+
+/* ---- Batched path: one persistent launch over many chunks of mixed resolution ------------- */
+
+#define UW_BATCH_MAX_RESOLUTION 32
+
+/* Launch shape of the persistent kernel (chosen by the autotuner, see autotune.py).
+ * block: threads per workgroup (128, 256 or 512); grid: persistent workgroups (0 = as many as
+ * fit on the device); slab[n]: cell planes per x-slab for chunks of n cells per axis (n = slab[n]
+ * builds the chunk in one slab with one density evaluation; smaller slabs bound LDS use and
+ * evaluate the densities twice). */
+typedef struct uw_batch_tuning {
+    int32_t block;
+    int32_t grid;
+    int8_t slab[UW_BATCH_MAX_RESOLUTION + 1];
+} uw_batch_tuning;
+
+int uw_get_batch_tuning(const uw_ctx* ctx, uw_batch_tuning* t);
+int uw_set_batch_tuning(uw_ctx* ctx, const uw_batch_tuning* t);
+
+/* Replaces the per-frame one-chunk-at-a-time loop of World::update (world.rs:93-145) with one
+ * build of a whole chunk set. coords: device int32[3 n], res: device int32[n] cells per axis
+ * (1..32, per chunk). Output as for uw_build_chunks_plain; chunk ranges are placed in completion
+ * order (each chunk's record gives its offsets; contents are deterministic). Asynchronous on
+ * `stream`; no host synchronisation, so the call can be captured in a graph. */
+int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                            const uw_mesh_out* out, void* stream);
+
+/* The batched build captured as an instantiated HIP graph over fixed buffers; relaunch it after
+ * changing the contents (not the addresses or count) of coords/res. */
+typedef struct uw_graph uw_graph;
+int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                            const uw_mesh_out* out, uw_graph** g);
+int uw_graph_launch(uw_graph* g, void* stream);
+void uw_graph_destroy(uw_graph* g);
 
 #ifdef __cplusplus
 }

@@ -5,6 +5,8 @@ include/uwmesh.h (libuwmesh.so); this package is the host-side mirror of the ref
 interface (noise::Perlin + Chunk) and the device-resident multi-chunk forms.
 """
 from . import params
-from .engine import (Chunk, MeshBatch, Perlin, build_chunks_plain, permutation_table, sample_density)
+from .engine import (BatchedGraph, Chunk, MeshBatch, Perlin, batch_capacity, build_chunks_batched,
+                     build_chunks_plain, get_batch_tuning, permutation_table, sample_density, set_batch_tuning)
 
-__all__ = ["params", "Perlin", "Chunk", "MeshBatch", "build_chunks_plain", "sample_density", "permutation_table"]
+__all__ = ["params", "Perlin", "Chunk", "MeshBatch", "BatchedGraph", "build_chunks_plain", "build_chunks_batched",
+           "batch_capacity", "sample_density", "permutation_table", "get_batch_tuning", "set_batch_tuning"]

@@ -55,6 +55,15 @@ class UwTotals(ctypes.Structure):
                 ("overflow", ctypes.c_uint32)]
 
 
+BATCH_MAX_RESOLUTION = 32
+CHUNK_INVALID = 8
+
+
+class UwBatchTuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32),
+                ("slab", ctypes.c_int8 * (BATCH_MAX_RESOLUTION + 1))]
+
+
 class UwMeshOut(ctypes.Structure):
     _fields_ = [("verts", c_void_p), ("inds", c_void_p), ("info", c_void_p), ("totals", c_void_p),
                 ("vert_cap", ctypes.c_uint64), ("ind_cap", ctypes.c_uint64)]
@@ -68,6 +77,8 @@ SIGNATURES = {
     "uw_last_error": (ctypes.c_char_p, [c_void_p]),
     "uw_ctx_permutation": (ctypes.c_int, [c_void_p, P(ctypes.c_uint8)]),
     "uw_permutation_table": (None, [ctypes.c_uint32, P(ctypes.c_uint8)]),
+    "uw_profile_enable": (ctypes.c_int, [c_void_p, ctypes.c_int]),
+    "uw_profile_read": (ctypes.c_int, [c_void_p, P(ctypes.c_double), P(ctypes.c_uint64)]),
     "uw_max_verts": (ctypes.c_uint64, [ctypes.c_int32]),
     "uw_max_inds": (ctypes.c_uint64, [ctypes.c_int32]),
     "uw_sample_density": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p,
@@ -76,6 +87,13 @@ SIGNATURES = {
                                              c_void_p, c_void_p]),
     "uw_chunk_build_host": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            c_void_p, ctypes.c_uint64, c_void_p, ctypes.c_uint64, P(UwChunkInfo)]),
+    "uw_get_batch_tuning": (ctypes.c_int, [c_void_p, P(UwBatchTuning)]),
+    "uw_set_batch_tuning": (ctypes.c_int, [c_void_p, P(UwBatchTuning)]),
+    "uw_build_chunks_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut), c_void_p]),
+    "uw_batched_graph_create": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut),
+                                               P(c_void_p)]),
+    "uw_graph_launch": (ctypes.c_int, [c_void_p, c_void_p]),
+    "uw_graph_destroy": (None, [c_void_p]),
 }
 
 _lib = None

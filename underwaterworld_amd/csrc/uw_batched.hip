@@ -1,0 +1,570 @@
+// Batched path: thousands of chunks of mixed resolution meshed by ONE persistent launch.
+//
+// A fixed grid of workgroups (a few per CU) pulls chunk tasks from a global queue (atomic head
+// over a resolution-sorted order, largest chunks first). Each workgroup builds its chunk entirely
+// in LDS:
+//   1. per-axis octave tables of the chunk (uw_build_chunk_tables);
+//   2. densities, x-slab by x-slab, by the line evaluator (uw_noise_planes) into a ring of
+//      density planes;
+//   3. a counting pass over the cells (cube case -> owned vertex records, triangles);
+//   4. one atomic reservation of the chunk's contiguous vertex and index ranges;
+//   5. emission: rounds of cells are classified and block-scanned; active cells are compacted
+//      into an LDS list, and the list (flushed when full or at the end of a slab) drives the edge
+//      lerps and the index writes. Vertex ids follow the reference's first-seen order through the
+//      edge-ownership rule (uw_mesh.h); each owner's vertex base and case sit in an LDS ring of the
+//      last cells.
+// Chunks that fit one slab compute their densities once; larger chunks recompute them in step 5
+// (the densities are not stored anywhere in HBM). The launch sequence (reset, order, persistent
+// kernel) has no host synchronisation, so it can be captured in a hipGraph.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "uw_mesh.h"
+#include "uw_noise.h"
+
+namespace {
+
+constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
+constexpr int kMaxL = kMaxN + 1;
+constexpr int kLPad = (kMaxL + 3) & ~3;  // ADJ/POST rows padded to keep later tables 16-byte aligned
+constexpr int kRing = 2048;       // cell ring (vertex base << 8 | case), power of two
+constexpr int kListCap = 512;     // compacted active cells pending emission
+constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
+
+struct BatchArgs {
+    uw_field f;
+    const uw_tables* tab;
+    const int32_t* coords;
+    const int32_t* res;
+    const uint32_t* order;
+    int32_t nchunks;
+    uint32_t* head;
+    uw_chunk_info* info;
+    uw_totals* totals;
+    float* verts;
+    uint32_t* inds;
+    uint64_t vcap, icap;
+    int8_t sx[kMaxN + 1]; // cell planes per slab for each resolution
+};
+
+__global__ void k_reset(uint32_t* head, uw_totals* totals) {
+    if (threadIdx.x == 0) {
+        *head = 0u;
+        uw_totals z = {0, 0, 0, 0};
+        *totals = z;
+    }
+}
+
+// Counting sort of chunk indices by resolution, largest first (one workgroup).
+__global__ void __launch_bounds__(1024) k_order(const int32_t* __restrict__ res, int32_t nchunks,
+                                                uint32_t* __restrict__ order) {
+    __shared__ uint32_t cnt[kMaxN + 2];
+    for (int i = threadIdx.x; i < kMaxN + 2; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < nchunks; i += blockDim.x) {
+        const int32_t n = res[i];
+        const int b = (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1;  // descending n, invalid last
+        atomicAdd(&cnt[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t run = 0;
+        for (int b = 0; b < kMaxN + 2; ++b) {
+            const uint32_t c = cnt[b];
+            cnt[b] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    for (int32_t i = threadIdx.x; i < nchunks; i += blockDim.x) {
+        const int32_t n = res[i];
+        const int b = (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1;
+        order[atomicAdd(&cnt[b], 1u)] = uint32_t(i);
+    }
+}
+
+template <int BLOCK>
+__device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long v, unsigned long long* total,
+                                                             unsigned long long* lds) {
+    constexpr int W = BLOCK / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned long long inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned long long t = __shfl_up(inc, d, 64);
+        if (lane >= d) inc += t;
+    }
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    unsigned long long base = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        const unsigned long long t = lds[w];
+        base += (w < wave) ? t : 0ull;
+        sum += t;
+    }
+    __syncthreads();
+    *total = sum;
+    return base + inc - v;
+}
+
+struct CellCounts {
+    uint32_t cs, nv, nt, act;
+};
+
+// Corner densities of cell (x, y, z) from the density-plane ring (plane x at slot x % ns).
+__device__ __forceinline__ void cell_corners(const float* ring, int32_t ns, int32_t L, int32_t x, int32_t y,
+                                             int32_t z, float r[8]) {
+    const int32_t s0 = x % ns;
+    const int32_t s1 = (s0 + 1 == ns) ? 0 : s0 + 1;
+    const float* p0 = ring + s0 * L * L + y * L + z;
+    const float* p1 = ring + s1 * L * L + y * L + z;
+    r[0] = p0[0];
+    r[1] = p1[0];
+    r[2] = p1[1];
+    r[3] = p0[1];
+    r[4] = p0[L];
+    r[5] = p1[L];
+    r[6] = p1[L + 1];
+    r[7] = p0[L + 1];
+}
+
+__device__ __forceinline__ CellCounts classify_cell(const float* ring, int32_t ns, int32_t L, int32_t x, int32_t y,
+                                                    int32_t z, float iso, const uint32_t* lut) {
+    float r[8];
+    cell_corners(ring, ns, L, x, y, z, r);
+    CellCounts c;
+    c.cs = uw_case_of(r, iso);
+    const uint32_t w = lut[c.cs];
+    c.nv = __popc(w & 0xFFFu & uw_owned_mask(x, y, z));
+    c.nt = w >> 16;
+    c.act = (c.cs != 0u && c.cs != 255u) ? 1u : 0u;
+    return c;
+}
+
+// LDS carve-up (bytes): fixed part, then the per-octave axis tables, then the density ring.
+__host__ __device__ constexpr size_t lds_fixed_bytes() {
+    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + size_t(kRing) * 4 + 2 * size_t(kListCap) * 4 +
+           2 * kLPad * 4 /*ADJ, POST*/ + 256 /*P*/;
+}
+__host__ __device__ inline size_t lds_table_bytes(int octaves) { return 3 * size_t(octaves) * kMaxL * sizeof(uw_axis); }
+
+template <int BLOCK, int OMAX>
+__global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int32_t O = a.f.octaves;
+    float4* GP = reinterpret_cast<float4*>(smem);
+    uint64_t* TRI = reinterpret_cast<uint64_t*>(GP + 256);
+    uint32_t* LUT = reinterpret_cast<uint32_t*>(TRI + 256);
+    uint32_t* RING = LUT + 256;
+    uint32_t* LIST = RING + kRing;
+    uint32_t* LIOFF = LIST + kListCap;
+    float* ADJ = reinterpret_cast<float*>(LIOFF + kListCap);
+    float* POST = ADJ + kLPad;
+    uint8_t* P = reinterpret_cast<uint8_t*>(POST + kLPad);
+    uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);
+    uw_axis* AY = AX + O * kMaxL;
+    uw_axis* AZ = AY + O * kMaxL;
+    float* DENS = reinterpret_cast<float*>(AZ + O * kMaxL);
+    __shared__ unsigned long long scan_lds[BLOCK / 64];
+    __shared__ uint32_t s_task;
+    __shared__ unsigned long long s_vo, s_io;
+    __shared__ int s_emit;
+
+    const int tid = threadIdx.x;
+    const uw_tables* tab = a.tab;
+    for (int i = tid; i < 256; i += BLOCK) {
+        const uint8_t p = tab->perm[i];
+        P[i] = p;
+        GP[i] = tab->grad[p & 15];
+        TRI[i] = tab->tri[i];
+        LUT[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
+    }
+    const float iso = a.f.iso;
+
+    for (;;) {
+        if (tid == 0) s_task = atomicAdd(a.head, 1u);
+        __syncthreads();
+        const uint32_t task = s_task;
+        if (task >= uint32_t(a.nchunks)) break;
+        const int32_t c = int32_t(a.order[task]);
+        const int32_t n = a.res[c];
+        if (n < 1 || n > kMaxN || O > OMAX) {
+            if (tid == 0) {
+                uw_chunk_info r = {UW_CHUNK_INVALID | UW_CHUNK_BLANK, 0, 0, 0, 0, 0};
+                a.info[c] = r;
+            }
+            __syncthreads();
+            continue;
+        }
+        const int32_t L = n + 1, nn = n * n;
+        const uw_divider dn = uw_make_divider(uint32_t(n)), dnn = uw_make_divider(uint32_t(nn));
+        const int32_t cx = a.coords[3 * c], cy = a.coords[3 * c + 1], cz = a.coords[3 * c + 2];
+        const float step = float(a.f.extent) / float(n);
+        // chunk tables use row stride L (not kMaxL): [O][L] per axis
+        const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
+        uw_build_chunk_tables(a.f, P, cx, cy, cz, L, step, T, tid, BLOCK);
+        __syncthreads();
+
+        const int32_t sx = a.sx[n];
+        const int32_t ns = sx + 1;
+        const int32_t nslab = (n + sx - 1) / sx;
+        const bool single = nslab == 1;
+
+        // ---- counting pass ----
+        int below = 0;
+        unsigned long long cnt = 0;  // nv | nt << 24 | act << 48 (per thread)
+        for (int32_t s = 0; s < nslab; ++s) {
+            const int32_t xa = s * sx, xe = min(n, xa + sx);
+            below |= uw_noise_planes<OMAX>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+            __syncthreads();
+            for (int32_t g = xa * nn + tid; g < xe * nn; g += BLOCK) {
+                int32_t x, y, z;
+                uw_cell_xyz(uint32_t(g), dn, dnn, &x, &y, &z);
+                const CellCounts cc = classify_cell(DENS, ns, L, x, y, z, iso, LUT);
+                cnt += (unsigned long long)cc.nv | ((unsigned long long)cc.nt << 24) |
+                       ((unsigned long long)cc.act << 48);
+            }
+            if (!single) __syncthreads();
+        }
+        below = __syncthreads_or(below);
+        unsigned long long tot;
+        block_scan_u64<BLOCK>(cnt, &tot, scan_lds);
+        const uint32_t nv = uint32_t(tot & 0xFFFFFFull), nt = uint32_t((tot >> 24) & 0xFFFFFFull),
+                       na = uint32_t(tot >> 48);
+        const uint32_t ni = 3u * nt;
+        if (tid == 0) {
+            uint32_t flags = 0;
+            unsigned long long vo = 0, io = 0;
+            if (!below) flags = UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP;
+            else if (ni == 0) flags = UW_CHUNK_BLANK;
+            if (ni > 0) {
+                vo = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_verts), (unsigned long long)nv);
+                io = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_inds), (unsigned long long)ni);
+                atomicAdd(&a.totals->n_nonblank, 1u);
+                if (vo + nv > a.vcap || io + ni > a.icap) {
+                    flags |= UW_CHUNK_OVERFLOW;
+                    atomicOr(&a.totals->overflow, 1u);
+                }
+            }
+            uw_chunk_info r = {flags, nv, ni, na, vo, io};
+            a.info[c] = r;
+            s_vo = vo;
+            s_io = io;
+            s_emit = (ni > 0 && !(flags & UW_CHUNK_OVERFLOW)) ? 1 : 0;
+        }
+        __syncthreads();
+        if (!s_emit) continue;
+
+        // ---- emission pass ----
+        const unsigned long long vo = s_vo, io = s_io;
+        const float ox = float(a.f.extent * cx), oy = float(a.f.extent * cy), oz = float(a.f.extent * cz);
+        // rounds that may stay pending before the cell ring would overwrite an owner
+        const int32_t max_pending_rounds = max(1, (kRing - nn - 1) / BLOCK - 1);
+        uint32_t carry_v = 0, carry_i = 0;
+        for (int32_t s = 0; s < nslab; ++s) {
+            const int32_t xa = s * sx, xe = min(n, xa + sx);
+            if (!single) {
+                uw_noise_planes<OMAX>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+                __syncthreads();
+            }
+            uint32_t list_n = 0;
+            int32_t pending_rounds = 0;
+            const int32_t gend = xe * nn;
+            for (int32_t g0 = xa * nn; g0 < gend; g0 += BLOCK) {
+                const int32_t g = g0 + tid;
+                CellCounts cc = {0, 0, 0, 0};
+                if (g < gend) {
+                    int32_t x, y, z;
+                    uw_cell_xyz(uint32_t(g), dn, dnn, &x, &y, &z);
+                    cc = classify_cell(DENS, ns, L, x, y, z, iso, LUT);
+                }
+                const unsigned long long packed = (unsigned long long)cc.nv | ((unsigned long long)cc.nt << 16) |
+                                                  ((unsigned long long)cc.act << 32);
+                unsigned long long rt;
+                const unsigned long long pre = block_scan_u64<BLOCK>(packed, &rt, scan_lds);
+                if (g < gend) {
+                    RING[g & (kRing - 1)] = ((carry_v + uint32_t(pre & 0xFFFFull)) << 8) | cc.cs;
+                    if (cc.act) {
+                        const uint32_t slot = list_n + uint32_t(pre >> 32);
+                        LIST[slot] = uint32_t(g);
+                        LIOFF[slot] = carry_i + 3u * uint32_t((pre >> 16) & 0xFFFFull);
+                    }
+                }
+                carry_v += uint32_t(rt & 0xFFFFull);
+                carry_i += 3u * uint32_t((rt >> 16) & 0xFFFFull);
+                list_n += uint32_t(rt >> 32);
+                ++pending_rounds;
+                const bool last = g0 + BLOCK >= gend;
+                if (last || list_n > uint32_t(kListCap - BLOCK) || pending_rounds >= max_pending_rounds) {
+                    __syncthreads();
+                    for (uint32_t e = tid; e < list_n; e += BLOCK) {
+                        const int32_t gg = int32_t(LIST[e]);
+                        const uint32_t w = RING[gg & (kRing - 1)];
+                        const uint32_t cs = w & 255u, myv = w >> 8;
+                        int32_t x, y, z;
+                        uw_cell_xyz(uint32_t(gg), dn, dnn, &x, &y, &z);
+                        float r[8];
+                        cell_corners(DENS, ns, L, x, y, z, r);
+                        const uint32_t owned = uw_owned_mask(x, y, z);
+                        const uint64_t row = TRI[cs];
+                        const int nidx = 3 * int(row & 15u);
+                        const uint64_t vb = vo + myv;
+                        const uint64_t ib = io + LIOFF[e];
+                        uint32_t seen = 0;
+                        for (int j = 0; j < nidx; ++j) {
+                            const uint32_t ed = uint32_t(row >> (4 + 4 * j)) & 15u;
+                            const uint32_t bit = 1u << ed;
+                            uint32_t id;
+                            if (owned & bit) {
+                                const uint32_t rank = __popc(uint32_t(tab->before[cs][ed]) & owned);
+                                id = myv + rank;
+                                if (!(seen & bit)) {
+                                    float p[3];
+                                    uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
+                                    float* dst = a.verts + 3 * (vb + rank);
+                                    dst[0] = p[0];
+                                    dst[1] = p[1];
+                                    dst[2] = p[2];
+                                }
+                            } else {
+                                int32_t qx, qy, qz;
+                                uint32_t qe;
+                                uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
+                                const int32_t q = (qx * n + qy) * n + qz;
+                                const uint32_t qw = RING[q & (kRing - 1)];
+                                id = (qw >> 8) +
+                                     __popc(uint32_t(tab->before[qw & 255u][qe]) & uw_owned_mask(qx, qy, qz));
+                            }
+                            seen |= bit;
+                            a.inds[ib + j] = id;
+                        }
+                    }
+                    __syncthreads();
+                    list_n = 0;
+                    pending_rounds = 0;
+                }
+            }
+        }
+    }
+}
+
+using BatchedKernel = void (*)(BatchArgs);
+
+template <int BLOCK>
+BatchedKernel pick_octaves(int omax) {
+    if (omax <= 4) return k_batched<BLOCK, 4>;
+    if (omax <= 8) return k_batched<BLOCK, 8>;
+    return k_batched<BLOCK, 16>;
+}
+
+BatchedKernel pick_kernel(int block, int omax) {
+    switch (block) {
+        case 128: return pick_octaves<128>(omax);
+        case 512: return pick_octaves<512>(omax);
+        default: return pick_octaves<256>(omax);
+    }
+}
+
+int omax_of(int octaves) { return octaves <= 4 ? 4 : (octaves <= 8 ? 8 : 16); }
+
+size_t dens_bytes_for(const uw_batch_tuning& t) {
+    size_t m = 0;
+    for (int n = 1; n <= kMaxN; ++n) {
+        const size_t L = size_t(n) + 1;
+        const size_t ns = size_t(t.slab[n]) + 1;
+        m = ns * L * L * 4 > m ? ns * L * L * 4 : m;
+    }
+    return m;
+}
+
+}  // namespace
+
+// Default tuning: whole-chunk slabs while the (n+1)^3 densities fit the slab budget, else the
+// largest slab that does.
+void uw_default_batch_tuning(uw_batch_tuning* t) {
+    memset(t, 0, sizeof(*t));
+    t->block = 256;
+    t->grid = 0;
+    for (int n = 1; n <= kMaxN; ++n) {
+        const int L = n + 1;
+        int sx = n;
+        if (L * L * L * 4 > kDensBudget) sx = max(1, kDensBudget / (4 * L * L) - 1);
+        t->slab[n] = int8_t(min(sx, n));
+    }
+}
+
+int uw_check_batch_tuning(const uw_batch_tuning& t) {
+    if (t.block != 128 && t.block != 256 && t.block != 512) return UW_ERR_INVALID_ARG;
+    if (t.grid < 0) return UW_ERR_INVALID_ARG;
+    for (int n = 1; n <= kMaxN; ++n)
+        if (t.slab[n] < 1 || t.slab[n] > n) return UW_ERR_INVALID_ARG;
+    return UW_OK;
+}
+
+namespace {
+
+// Everything a batched launch needs that must not happen inside stream capture: buffer
+// allocation, the kernel's LDS attribute and the persistent grid size.
+struct BatchLaunch {
+    BatchedKernel k;
+    size_t lds;
+    int grid;
+};
+
+int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
+    const uw_batch_tuning& tune = ctx->batch_tuning;
+    const int32_t octaves = ctx->params.octaves;
+    if (size_t(nchunks) > ctx->batch_order_cap) {
+        (void)hipFree(ctx->d_batch_order);
+        ctx->d_batch_order = nullptr;
+        ctx->batch_order_cap = 0;
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, size_t(nchunks) * sizeof(uint32_t)));
+        ctx->batch_order_cap = size_t(nchunks);
+    }
+    if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, sizeof(uint32_t)));
+    bl->k = pick_kernel(tune.block, omax_of(octaves));
+    bl->lds = lds_fixed_bytes() + lds_table_bytes(octaves) + dens_bytes_for(tune);
+    UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(bl->k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(bl->lds)));
+    bl->grid = tune.grid;
+    if (bl->grid == 0) {
+        int per_cu = 0;
+        UW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bl->k),
+                                                                 tune.block, bl->lds));
+        if (per_cu < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "batched kernel does not fit on a CU with this tuning");
+        bl->grid = per_cu * ctx->num_cus;
+    }
+    return UW_OK;
+}
+
+int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                   const uw_mesh_out* out, uint32_t* head, uint32_t* order, hipStream_t st) {
+    const uw_batch_tuning& tune = ctx->batch_tuning;
+    BatchArgs a;
+    memset(&a, 0, sizeof(a));
+    a.f = uw_make_field(ctx->params);
+    a.tab = ctx->d_tables;
+    a.coords = coords;
+    a.res = res;
+    a.nchunks = nchunks;
+    a.info = out->info;
+    a.totals = out->totals;
+    a.verts = out->verts;
+    a.inds = out->inds;
+    a.vcap = out->vert_cap;
+    a.icap = out->ind_cap;
+    for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
+    a.order = order;
+    a.head = head;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, head, a.totals);
+    hipLaunchKernelGGL(k_order, dim3(1), dim3(1024), 0, st, res, nchunks, order);
+    if (int rc = uw_prof_start(ctx, st)) return rc;
+    hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
+    UW_HIP(ctx, hipGetLastError());
+    return UW_OK;
+}
+
+bool args_ok(int32_t nchunks, const int32_t* coords, const int32_t* res, const uw_mesh_out* out) {
+    return nchunks >= 0 && out && out->info && out->totals && (nchunks == 0 || (coords && res)) &&
+           (out->vert_cap == 0 || out->verts) && (out->ind_cap == 0 || out->inds);
+}
+
+}  // namespace
+
+extern "C" {
+
+int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                            const uw_mesh_out* out, void* stream) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    if (!args_ok(nchunks, coords, res, out)) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    BatchLaunch bl;
+    if (int rc = prepare_batched(ctx, nchunks > 0 ? nchunks : 1, &bl)) return rc;
+    if (nchunks == 0) {
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctx->d_batch_head, out->totals);
+        UW_HIP(ctx, hipGetLastError());
+        return UW_OK;
+    }
+    return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, st);
+}
+
+int uw_get_batch_tuning(const uw_ctx* ctx, uw_batch_tuning* t) {
+    if (!ctx || !t) return UW_ERR_INVALID_ARG;
+    *t = ctx->batch_tuning;
+    return UW_OK;
+}
+
+int uw_set_batch_tuning(uw_ctx* ctx, const uw_batch_tuning* t) {
+    if (!ctx || !t) return UW_ERR_INVALID_ARG;
+    if (uw_check_batch_tuning(*t)) return uw_fail(ctx, UW_ERR_INVALID_ARG, "invalid batch tuning");
+    ctx->batch_tuning = *t;
+    return UW_OK;
+}
+
+int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                            const uw_mesh_out* out, uw_graph** g) {
+    if (!ctx || !g) return UW_ERR_INVALID_ARG;
+    *g = nullptr;
+    if (nchunks < 1 || !args_ok(nchunks, coords, res, out)) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    BatchLaunch bl;
+    if (int rc = prepare_batched(ctx, nchunks, &bl)) return rc;
+    // the graph owns its queue head and order buffers: the context's may be reallocated later
+    uint32_t* gbuf = nullptr;
+    UW_HIP(ctx, hipMalloc(&gbuf, (size_t(nchunks) + 1) * sizeof(uint32_t)));
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipFree(gbuf);
+        return uw_fail(ctx, UW_ERR_HIP, "hipStreamCreateWithFlags failed");
+    }
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc = UW_OK;
+    const bool was_profiling = ctx->profiling;
+    ctx->profiling = false;  // no timing events inside a graph
+    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gbuf + 1, cs);
+    ctx->profiling = was_profiling;
+    hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &graph) : hipSuccess;
+    (void)hipStreamDestroy(cs);
+    if (e != hipSuccess || rc != UW_OK || e2 != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipFree(gbuf);
+        if (e != hipSuccess) return uw_hip_fail(ctx, e, "hipStreamBeginCapture");
+        if (rc != UW_OK) return rc;
+        return uw_hip_fail(ctx, e2, "hipStreamEndCapture");
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        (void)hipFree(gbuf);
+        return uw_hip_fail(ctx, e, "hipGraphInstantiate");
+    }
+    uw_graph* out_g = new uw_graph();
+    out_g->graph = graph;
+    out_g->exec = exec;
+    out_g->device = ctx->device;
+    out_g->buffers = gbuf;
+    *g = out_g;
+    return UW_OK;
+}
+
+int uw_graph_launch(uw_graph* g, void* stream) {
+    if (!g) return UW_ERR_INVALID_ARG;
+    if (hipSetDevice(g->device) != hipSuccess) return UW_ERR_HIP;
+    return hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)) == hipSuccess ? UW_OK : UW_ERR_HIP;
+}
+
+void uw_graph_destroy(uw_graph* g) {
+    if (!g) return;
+    (void)hipSetDevice(g->device);
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    (void)hipFree(g->buffers);
+    delete g;
+}
+
+}  // extern "C"

@@ -16,6 +16,26 @@ int uw_hip_fail(uw_ctx* ctx, hipError_t e, const char* what) {
     return uw_fail(ctx, e == hipErrorOutOfMemory ? UW_ERR_OOM : UW_ERR_HIP, m);
 }
 
+int uw_prof_start(uw_ctx* ctx, hipStream_t st) {
+    if (!ctx->profiling) return UW_OK;
+    if (ctx->prof_used + 2 > ctx->prof_events.size()) {
+        for (int i = 0; i < 2; ++i) {
+            hipEvent_t e;
+            UW_HIP(ctx, hipEventCreate(&e));
+            ctx->prof_events.push_back(e);
+        }
+    }
+    UW_HIP(ctx, hipEventRecord(ctx->prof_events[ctx->prof_used], st));
+    return UW_OK;
+}
+
+int uw_prof_end(uw_ctx* ctx, hipStream_t st) {
+    if (!ctx->profiling) return UW_OK;
+    UW_HIP(ctx, hipEventRecord(ctx->prof_events[ctx->prof_used + 1], st));
+    ctx->prof_used += 2;
+    return UW_OK;
+}
+
 namespace {
 
 // Marsaglia xor128 step as implemented by rand_xorshift 0.3 XorShiftRng::next_u32.
@@ -165,8 +185,12 @@ int uw_ctx_create(const uw_params* p, int device, uw_ctx** out) {
     if (!ctx) return UW_ERR_OOM;
     ctx->params = *p;
     ctx->device = device;
+    uw_default_batch_tuning(&ctx->batch_tuning);
     build_permutation(p->seed, ctx->perm);
     build_tables(ctx->host_tables, ctx->perm);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->num_cus = prop.multiProcessorCount;
     if (hipSetDevice(device) != hipSuccess ||
         hipMalloc(&ctx->d_tables, sizeof(uw_tables)) != hipSuccess ||
         hipMemcpy(ctx->d_tables, &ctx->host_tables, sizeof(uw_tables), hipMemcpyHostToDevice) != hipSuccess) {
@@ -193,6 +217,9 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(ctx->d_inds1);
     (void)hipFree(ctx->d_info1);
     (void)hipFree(ctx->d_totals1);
+    (void)hipFree(ctx->d_batch_head);
+    (void)hipFree(ctx->d_batch_order);
+    for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
     delete ctx;
 }
 
@@ -206,6 +233,27 @@ int uw_ctx_permutation(const uw_ctx* ctx, uint8_t* out256) {
 
 void uw_permutation_table(uint32_t seed, uint8_t* out256) {
     if (out256) build_permutation(seed, out256);
+}
+
+int uw_profile_enable(uw_ctx* ctx, int on) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    ctx->profiling = on != 0;
+    return UW_OK;
+}
+
+int uw_profile_read(uw_ctx* ctx, double* total_ms, uint64_t* launches) {
+    if (!ctx || !total_ms || !launches) return UW_ERR_INVALID_ARG;
+    double sum = 0.0;
+    for (size_t i = 0; i + 1 < ctx->prof_used; i += 2) {
+        UW_HIP(ctx, hipEventSynchronize(ctx->prof_events[i + 1]));
+        float ms = 0.0f;
+        UW_HIP(ctx, hipEventElapsedTime(&ms, ctx->prof_events[i], ctx->prof_events[i + 1]));
+        sum += ms;
+    }
+    *total_ms = sum;
+    *launches = ctx->prof_used / 2;
+    ctx->prof_used = 0;
+    return UW_OK;
 }
 
 uint64_t uw_max_verts(int32_t n) { return n < 1 ? 0 : 5ull * uint64_t(n) * uint64_t(n + 1) * uint64_t(n + 1); }

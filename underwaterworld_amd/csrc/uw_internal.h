@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/uwmesh.h"
 #include "mc_table.h"
@@ -38,13 +39,31 @@ struct uw_plain_scratch {
     uint32_t* ioff = nullptr;  // index offset of each compacted active cell
 };
 
+struct uw_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+    uint32_t* buffers = nullptr;  // queue head + chunk order owned by the graph
+};
+
+void uw_default_batch_tuning(uw_batch_tuning* t);
+
 struct uw_ctx {
     uw_params params;
     int device = 0;
+    int num_cus = 256;
+    uw_batch_tuning batch_tuning;
+    uint32_t* d_batch_head = nullptr;
+    uint32_t* d_batch_order = nullptr;
+    size_t batch_order_cap = 0;
     uint8_t perm[256];
     uw_tables host_tables;
     uw_tables* d_tables = nullptr;
     uw_plain_scratch plain;
+    // kernel timing (uw_profile_*)
+    bool profiling = false;
+    std::vector<hipEvent_t> prof_events;  // pairs (start, end), reused
+    size_t prof_used = 0;                 // events recorded since the last read
     // single-chunk host facade buffers
     int32_t* d_coord1 = nullptr;
     float* d_verts1 = nullptr;
@@ -56,6 +75,9 @@ struct uw_ctx {
 };
 
 int uw_fail(uw_ctx* ctx, int code, const std::string& msg);
+// Record the start / end event of a timed kernel on `st` (no-ops unless profiling is enabled).
+int uw_prof_start(uw_ctx* ctx, hipStream_t st);
+int uw_prof_end(uw_ctx* ctx, hipStream_t st);
 int uw_hip_fail(uw_ctx* ctx, hipError_t e, const char* what);
 
 #define UW_HIP(ctx, expr)                                        \

@@ -106,3 +106,119 @@ __device__ __forceinline__ float uw_density_from(float total, float amp_sum, flo
     const float p = total / amp_sum;
     return (adj + p) + post;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Line evaluation. Along a grid line of fixed (x, y) only z varies, and within one lattice cell of
+// octave k the eight corner gradients are fixed, so each corner term is affine in dz:
+//   g_ijl(dz) = Gx dx_i + Gy dy_j + Gz (dz - l) = P_ijl + Gz_ijl dz_l.
+// The x and y lerps are linear, so the two z faces of the cell reduce to
+//   N_l(dz) = A_l + B_l dz_l,  A_l = lerp_y(lerp_x(P_.. l)),  B_l = lerp_y(lerp_x(Gz_.. l)),
+// and a sample costs N0, N1 (two FMAs), one lerp by the z fade and the octave accumulation. The
+// A/B setup (eight gradient fetches and ~40 FLOPs) is redone only when the line enters a new
+// lattice cell of that octave, which is wave-uniform because every lane of a wave walks its own
+// line at the same z index.
+
+struct uw_line_oct {
+    float A0, B0, A1, B1;
+};
+
+// Per-line, per-octave setup for the lattice cell whose low/high z bytes are packed in zbits.
+// bb = b00 | b01 << 8 | b10 << 16 | b11 << 24 (second-level hashes of the line's x/y corners).
+__device__ __forceinline__ uw_line_oct uw_line_setup(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zbits,
+                                                     const float4* GP) {
+    const uint32_t z0 = zbits & 255u, z1 = (zbits >> 8) & 255u;
+    const uint32_t b00 = bb & 255u, b01 = (bb >> 8) & 255u, b10 = (bb >> 16) & 255u, b11 = bb >> 24;
+    const float4 G000 = GP[b00 ^ z0], G100 = GP[b10 ^ z0], G010 = GP[b01 ^ z0], G110 = GP[b11 ^ z0];
+    const float4 G001 = GP[b00 ^ z1], G101 = GP[b10 ^ z1], G011 = GP[b01 ^ z1], G111 = GP[b11 ^ z1];
+    const float P000 = fmaf(G000.y, Y.d, G000.x * X.d), P100 = fmaf(G100.y, Y.d, G100.x * X.dm1);
+    const float P010 = fmaf(G010.y, Y.dm1, G010.x * X.d), P110 = fmaf(G110.y, Y.dm1, G110.x * X.dm1);
+    const float P001 = fmaf(G001.y, Y.d, G001.x * X.d), P101 = fmaf(G101.y, Y.d, G101.x * X.dm1);
+    const float P011 = fmaf(G011.y, Y.dm1, G011.x * X.d), P111 = fmaf(G111.y, Y.dm1, G111.x * X.dm1);
+    const float a = X.fade, b = Y.fade;
+    uw_line_oct s;
+    s.A0 = uw_lerp(b, uw_lerp(a, P000, P100), uw_lerp(a, P010, P110));
+    s.A1 = uw_lerp(b, uw_lerp(a, P001, P101), uw_lerp(a, P011, P111));
+    s.B0 = uw_lerp(b, uw_lerp(a, G000.z, G100.z), uw_lerp(a, G010.z, G110.z));
+    s.B1 = uw_lerp(b, uw_lerp(a, G001.z, G101.z), uw_lerp(a, G011.z, G111.z));
+    return s;
+}
+
+__device__ __forceinline__ float uw_line_sample(const uw_line_oct& s, const uw_axis& Z) {
+    const float n0 = fmaf(s.B0, Z.d, s.A0);
+    const float n1 = fmaf(s.B1, Z.dm1, s.A1);
+    const float r = uw_lerp(Z.fade, n0, n1) * UW_PERLIN_SCALE;
+    return fminf(fmaxf(r, -1.0f), 1.0f);
+}
+
+// Per-chunk LDS tables: every (axis index, octave) entry of the chunk and the z-bias split.
+struct uw_chunk_tables {
+    uw_axis* AX;   // [O][L], hashed
+    uw_axis* AY;   // [O][L]
+    uw_axis* AZ;   // [O][L]
+    float* ADJ;    // [L]
+    float* POST;   // [L]
+};
+
+__device__ __forceinline__ void uw_build_chunk_tables(const uw_field& f, const uint8_t* perm, int32_t cx, int32_t cy,
+                                                      int32_t cz, int32_t L, float step, const uw_chunk_tables& T,
+                                                      int tid, int nthr) {
+    const int32_t O = f.octaves;
+    const int32_t nent = 3 * O * L;
+    for (int e = tid; e < nent; e += nthr) {
+        const int32_t axis = e / (O * L);
+        const int32_t r = e - axis * O * L;
+        const int32_t k = r / L, i = r - k * L;
+        const double fk = f.octave_freq[k];
+        if (axis == 0) T.AX[r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, cx, i, step), fk), perm);
+        else if (axis == 1) T.AY[r] = uw_axis_of(uw_sample_coord(f, cy, i, step), fk);
+        else T.AZ[r] = uw_axis_of(uw_sample_coord(f, cz, i, step), fk);
+    }
+    for (int i = tid; i < L; i += nthr) uw_bias_terms(f, uw_sample_coord(f, cz, i, step), &T.ADJ[i], &T.POST[i]);
+}
+
+// Densities of x-planes [xa, xb) of a chunk whose tables are in T, one thread per (x, y) line.
+// Plane x is written to dst + slot(x) * L * L with slot(x) = x % nslot, z fastest.
+// Returns, per thread, whether some sample was not above iso.
+template <int OMAX>
+__device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uint8_t* P,
+                                               const float4* GP, int32_t L, int32_t xa, int32_t xb, float* dst,
+                                               int32_t nslot, int tid, int nthr) {
+    const int32_t O = f.octaves;
+    const int32_t lines = (xb - xa) * L;
+    int below = 0;
+    for (int32_t t = tid; t < lines; t += nthr) {
+        const int32_t ix = xa + t / L, iy = t - (t / L) * L;
+        float* out = dst + (ix % nslot) * L * L + iy * L;
+        uint32_t bb[OMAX];
+        uw_line_oct st[OMAX];
+        uint32_t zcur[OMAX];
+#pragma unroll
+        for (int k = 0; k < OMAX; ++k) {
+            if (k < O) {
+                const uint32_t xb2 = T.AX[k * L + ix].bits, yb = T.AY[k * L + iy].bits;
+                const uint32_t a0 = xb2 & 255u, a1 = xb2 >> 8, y0 = yb & 255u, y1 = yb >> 8;
+                bb[k] = uint32_t(P[a0 ^ y0]) | (uint32_t(P[a0 ^ y1]) << 8) | (uint32_t(P[a1 ^ y0]) << 16) |
+                        (uint32_t(P[a1 ^ y1]) << 24);
+                zcur[k] = 0xFFFFFFFFu;
+            }
+        }
+        for (int32_t iz = 0; iz < L; ++iz) {
+            float total = 0.0f;
+#pragma unroll
+            for (int k = 0; k < OMAX; ++k) {
+                if (k < O) {
+                    const uw_axis Z = T.AZ[k * L + iz];
+                    if (Z.bits != zcur[k]) {
+                        st[k] = uw_line_setup(T.AX[k * L + ix], T.AY[k * L + iy], bb[k], Z.bits, GP);
+                        zcur[k] = Z.bits;
+                    }
+                    total = fmaf(f.amp[k], uw_line_sample(st[k], Z), total);
+                }
+            }
+            const float rho = uw_density_from(total, f.amp_sum, T.ADJ[iz], T.POST[iz]);
+            out[iz] = rho;
+            below |= !(rho > f.iso);
+        }
+    }
+    return below;
+}

@@ -26,24 +26,28 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------------------------
-// Density grid. One workgroup per (chunk, slab of `planes` x-planes). Per-axis octave data for the
-// slab's x range and the full y, z ranges are built once in LDS, then every sample combines them.
+// Density grid. One workgroup per (chunk, slab of `planes` x-planes): the chunk's per-axis octave
+// tables are built in LDS, the slab's densities are evaluated line by line (uw_noise_planes) into
+// LDS and then stored to HBM with contiguous, coalesced writes.
+template <int OMAX>
 __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_tables* __restrict__ tab,
                                                          const int32_t* __restrict__ coords, int32_t n,
                                                          float step, int32_t planes, int32_t slabs,
                                                          float* __restrict__ dens, uint32_t* __restrict__ below) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int32_t L = n + 1, O = f.octaves;
+    const int32_t Lp = (L + 3) & ~3;
     const int32_t chunk = blockIdx.x / slabs, slab = blockIdx.x % slabs;
     const int32_t x0 = slab * planes;
     const int32_t nx = min(planes, L - x0);
     float4* GP = reinterpret_cast<float4*>(smem);                  // 256 x 16 B
-    uint8_t* P = smem + 256 * sizeof(float4);                     // 256 B
-    uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);             // [O][planes]
-    uw_axis* AY = AX + O * planes;                                 // [O][L]
-    uw_axis* AZ = AY + O * L;                                      // [O][L]
-    float* ADJ = reinterpret_cast<float*>(AZ + O * L);             // [L]
-    float* POST = ADJ + L;                                         // [L]
+    float* ADJ = reinterpret_cast<float*>(GP + 256);               // Lp
+    float* POST = ADJ + Lp;                                        // Lp
+    uint8_t* P = reinterpret_cast<uint8_t*>(POST + Lp);            // 256 B
+    uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);             // [O][L] x3
+    uw_axis* AY = AX + O * L;
+    uw_axis* AZ = AY + O * L;
+    float* SLAB = reinterpret_cast<float*>(AZ + O * L);            // [planes][L][L]
 
     const int32_t cx = coords[3 * chunk], cy = coords[3 * chunk + 1], cz = coords[3 * chunk + 2];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
@@ -51,42 +55,17 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
         P[i] = p;
         GP[i] = tab->grad[p & 15];
     }
-    const int32_t nent = O * (nx + 2 * L);
-    for (int e = threadIdx.x; e < nent; e += blockDim.x) {
-        const int32_t k = e % O;
-        const int32_t r = e / O;
-        const double fk = f.octave_freq[k];
-        if (r < nx) {
-            AX[k * planes + r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, cx, x0 + r, step), fk), tab->perm);
-        } else if (r < nx + L) {
-            const int32_t i = r - nx;
-            AY[k * L + i] = uw_axis_of(uw_sample_coord(f, cy, i, step), fk);
-        } else {
-            const int32_t i = r - nx - L;
-            AZ[k * L + i] = uw_axis_of(uw_sample_coord(f, cz, i, step), fk);
-        }
-    }
-    for (int i = threadIdx.x; i < L; i += blockDim.x) uw_bias_terms(f, uw_sample_coord(f, cz, i, step), &ADJ[i], &POST[i]);
     __syncthreads();
-
+    const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
+    uw_build_chunk_tables(f, P, cx, cy, cz, L, step, T, threadIdx.x, blockDim.x);
+    __syncthreads();
+    // plane x lands in slot x % planes = x - x0
+    int any_below = uw_noise_planes<OMAX>(f, T, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
+    __syncthreads();
     const int32_t LL = L * L;
     const int32_t count = nx * LL;
-    const uint64_t base = uint64_t(chunk) * uint64_t(LL) * uint64_t(L) + uint64_t(x0) * uint64_t(LL);
-    int any_below = 0;
-    for (int32_t s = threadIdx.x; s < count; s += blockDim.x) {
-        const int32_t ix = s / LL;
-        const int32_t rem = s - ix * LL;
-        const int32_t iy = rem / L;
-        const int32_t iz = rem - iy * L;
-        float total = 0.0f;
-        for (int32_t k = 0; k < O; ++k) {
-            const float v = uw_perlin_octave(AX[k * planes + ix], AY[k * L + iy], AZ[k * L + iz], P, GP);
-            total = fmaf(f.amp[k], v, total);
-        }
-        const float rho = uw_density_from(total, f.amp_sum, ADJ[iz], POST[iz]);
-        dens[base + s] = rho;
-        any_below |= !(rho > f.iso);
-    }
+    float* dst = dens + uint64_t(chunk) * uint64_t(LL) * uint64_t(L) + uint64_t(x0) * uint64_t(LL);
+    for (int32_t s = threadIdx.x; s < count; s += blockDim.x) dst[s] = SLAB[s];
     any_below = __syncthreads_or(any_below);
     if (threadIdx.x == 0 && any_below) atomicOr(&below[chunk], 1u);
 }
@@ -299,12 +278,24 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
 
 // LDS bytes of k_noise_grid.
 size_t noise_lds_bytes(int32_t O, int32_t planes, int32_t L) {
-    return 256 * sizeof(float4) + 256 + sizeof(uw_axis) * size_t(O) * size_t(planes + 2 * L) + 2 * sizeof(float) * L;
+    const size_t Lp = size_t((L + 3) & ~3);
+    return 256 * sizeof(float4) + 2 * sizeof(float) * Lp + 256 + sizeof(uw_axis) * 3 * size_t(O) * size_t(L) +
+           sizeof(float) * size_t(planes) * size_t(L) * size_t(L);
 }
 
+// x-planes per workgroup: the whole chunk while it fits 16 KB of LDS, else fewer (at least one).
 int32_t noise_planes(int32_t L) {
-    int32_t p = (2048 + L * L - 1) / (L * L);
+    int32_t p = (16 * 1024) / (4 * L * L);
     return p < 1 ? 1 : (p > L ? L : p);
+}
+
+using NoiseKernel = void (*)(uw_field, const uw_tables*, const int32_t*, int32_t, float, int32_t, int32_t, float*,
+                             uint32_t*);
+
+NoiseKernel pick_noise(int octaves) {
+    if (octaves <= 4) return k_noise_grid<4>;
+    if (octaves <= 8) return k_noise_grid<8>;
+    return k_noise_grid<16>;
 }
 
 int ensure_plain_scratch(uw_ctx* ctx, int32_t nchunks, int32_t n) {
@@ -341,9 +332,14 @@ int launch_noise(uw_ctx* ctx, const uw_field& f, const int32_t* coords, int32_t 
     const size_t lds = noise_lds_bytes(f.octaves, planes, L);
     const uint64_t blocks = uint64_t(nchunks) * uint64_t(slabs);
     if (blocks > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
+    NoiseKernel k = pick_noise(f.octaves);
+    UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds)));
     hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((nchunks + 255) / 256)), dim3(256), 0, st, below, uint64_t(nchunks), 0u);
-    hipLaunchKernelGGL(k_noise_grid, dim3(uint32_t(blocks)), dim3(kThreads), lds, st, f, ctx->d_tables, coords, n,
-                       step, planes, slabs, dens, below);
+    if (int rc = uw_prof_start(ctx, st)) return rc;
+    hipLaunchKernelGGL(k, dim3(uint32_t(blocks)), dim3(kThreads), lds, st, f, ctx->d_tables, coords, n, step, planes,
+                       slabs, dens, below);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
 }

@@ -224,6 +224,98 @@ def build_chunks_plain(perlin: Perlin, coords, n: int, vert_cap: int | None = No
     return MeshBatch(verts, inds, info[: nch * 32], totals, cases)
 
 
+def _res_tensor(res, nch, device):
+    import torch
+    if isinstance(res, int):
+        return torch.full((max(nch, 1),), int(res), dtype=torch.int32, device=device)
+    if isinstance(res, torch.Tensor):
+        return res.to(device=device, dtype=torch.int32).contiguous()
+    return torch.as_tensor(np.asarray(res, np.int32), device=device)
+
+
+def batch_capacity(res) -> tuple[int, int]:
+    """Worst-case vertex and index counts of a chunk set (resolution per chunk)."""
+    r = np.asarray(res, np.int64)
+    return int((5 * r * (r + 1) ** 2).sum()), int((15 * r ** 3).sum())
+
+
+def build_chunks_batched(perlin: Perlin, coords, res, vert_cap: int | None = None, ind_cap: int | None = None,
+                         stream=None) -> MeshBatch:
+    """One persistent launch over a mixed-resolution chunk set (res: cells per axis per chunk)."""
+    import torch
+    dev = torch.device("cuda", perlin.device)
+    ct = _coords_tensor(coords, dev)
+    nch = ct.shape[0]
+    rt = _res_tensor(res, nch, dev)
+    if vert_cap is None or ind_cap is None:
+        vc, ic = batch_capacity(rt[:nch].cpu().numpy() if nch else [])
+        vert_cap = vc if vert_cap is None else vert_cap
+        ind_cap = ic if ind_cap is None else ind_cap
+    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev)
+    out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
+    rc = _lib.lib().uw_build_chunks_batched(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), ctypes.c_void_p(rt.data_ptr()),
+                                            nch, ctypes.byref(out), _stream_ptr(stream))
+    _lib.check(rc, perlin.ctx)
+    return MeshBatch(verts, inds, info[: nch * 32], totals, None)
+
+
+class BatchedGraph:
+    """The batched build captured once over fixed buffers and replayed (uw_batched_graph_create)."""
+
+    def __init__(self, perlin: Perlin, coords, res, vert_cap: int | None = None, ind_cap: int | None = None):
+        import torch
+        dev = torch.device("cuda", perlin.device)
+        self.perlin = perlin
+        self.coords = _coords_tensor(coords, dev)
+        nch = self.coords.shape[0]
+        self.res = _res_tensor(res, nch, dev)
+        if vert_cap is None or ind_cap is None:
+            vc, ic = batch_capacity(self.res[:nch].cpu().numpy())
+            vert_cap = vc if vert_cap is None else vert_cap
+            ind_cap = ic if ind_cap is None else ind_cap
+        verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev)
+        self.batch = MeshBatch(verts, inds, info[: nch * 32], totals, None)
+        self._out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
+        self._g = ctypes.c_void_p()
+        torch.cuda.synchronize()
+        rc = _lib.lib().uw_batched_graph_create(perlin.ctx, ctypes.c_void_p(self.coords.data_ptr()),
+                                                ctypes.c_void_p(self.res.data_ptr()), nch, ctypes.byref(self._out),
+                                                ctypes.byref(self._g))
+        _lib.check(rc, perlin.ctx)
+
+    def launch(self, stream=None) -> MeshBatch:
+        rc = _lib.lib().uw_graph_launch(self._g, _stream_ptr(stream))
+        if rc != _lib.UW_OK:
+            raise _lib.UwError(rc, "uw_graph_launch failed")
+        return self.batch
+
+    def close(self):
+        if self._g:
+            _lib.lib().uw_graph_destroy(self._g)
+            self._g = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def get_batch_tuning(perlin: Perlin) -> dict:
+    t = _lib.UwBatchTuning()
+    _lib.check(_lib.lib().uw_get_batch_tuning(perlin.ctx, ctypes.byref(t)), perlin.ctx)
+    return {"block": t.block, "grid": t.grid, "slab": [int(v) for v in t.slab]}
+
+
+def set_batch_tuning(perlin: Perlin, tuning: dict):
+    t = _lib.UwBatchTuning()
+    t.block = int(tuning["block"])
+    t.grid = int(tuning.get("grid", 0))
+    for i, v in enumerate(tuning["slab"]):
+        t.slab[i] = int(v)
+    _lib.check(_lib.lib().uw_set_batch_tuning(perlin.ctx, ctypes.byref(t)), perlin.ctx)
+
+
 def sample_density(perlin: Perlin, coords, n: int, stream=None):
     """Density grids (nchunks, (n+1)^3) f32 and the not-early-blank flags (nchunks,) as tensors."""
     import torch
