@@ -62,6 +62,7 @@ typedef struct uw_params {
     int32_t bias_kind;                    /* enum uw_bias_kind                               */
     float bias_a, bias_b, bias_c;         /* bias coefficients, see enum uw_bias_kind        */
     int32_t chunk_extent;                 /* world units per chunk edge (chunk.rs:5)         */
+    double offset[3];                     /* world-space shift of every sample (0 = reference) */
 } uw_params;
 
 /* Per-chunk result record (32 bytes). */
@@ -173,6 +174,41 @@ int uw_set_batch_tuning(uw_ctx* ctx, const uw_batch_tuning* t);
  * `stream`; no host synchronisation, so the call can be captured in a graph. */
 int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
                             const uw_mesh_out* out, void* stream);
+
+/* ---- Differentiation (no counterpart in the reference, which has no gradients) ------------ */
+
+/* Replace the octave amplitudes (octaves values) and/or the world-space sample offset (3 values)
+ * of the context's field; NULL leaves a part unchanged. */
+int uw_ctx_set_field(uw_ctx* ctx, const double* amplitudes, const double* offset);
+
+/* Per-vertex record of the forward build that the backward pass needs: the chunk-local sample
+ * indices (x (n+1)^2 + y (n+1) + z) of the edge's endpoints a -> b and their densities. */
+typedef struct uw_vertex_tape {
+    uint32_t a, b;
+    float rho_a, rho_b;
+} uw_vertex_tape;
+
+/* Forward: the batched build of uw_build_chunks_batched that also writes tape[v] for every
+ * vertex v it writes (tape: device, out->vert_cap entries). */
+int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                                 const uw_mesh_out* out, uw_vertex_tape* tape, void* stream);
+
+/* Backward (vector-Jacobian product) of the forward through the edge lerps to the densities and,
+ * with the octave noise recomputed at each vertex's endpoints (nothing but the tape is stored),
+ * to the octave amplitudes and the world-space sample offset. The cube cases are held fixed.
+ *   info        device records written by the forward (n_verts, vert_off per chunk)
+ *   tape        device tape of the forward
+ *   grad_verts  device float[3 V]: dL/d(vertex position), laid out like the forward's verts
+ *   grad_dens   device float[nchunks * dens_stride] or NULL: dL/d(density) per chunk sample,
+ *               accumulated (the caller zeroes it); dens_stride >= (n+1)^3 of every chunk
+ *   grad_amp    device double[nchunks * octaves] or NULL: per-chunk dL/dA_k (includes the
+ *               normalisation by the amplitude sum)
+ *   grad_off    device double[nchunks * 3] or NULL: per-chunk dL/d(offset)
+ * Summation order across threads is not fixed (atomics into grad_dens), so results can differ in
+ * the last bits between runs. */
+int uw_adjoint_backward(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                        const uw_chunk_info* info, const uw_vertex_tape* tape, const float* grad_verts,
+                        float* grad_dens, int64_t dens_stride, double* grad_amp, double* grad_off, void* stream);
 
 /* The batched build captured as an instantiated HIP graph over fixed buffers; relaunch it after
  * changing the contents (not the addresses or count) of coords/res. */

@@ -31,6 +31,7 @@ class UwoParams(ctypes.Structure):
         ("bias_b", ctypes.c_float),
         ("bias_c", ctypes.c_float),
         ("chunk_extent", ctypes.c_int32),
+        ("offset", ctypes.c_double * 3),
     ]
 
 
@@ -107,6 +108,8 @@ def make_params(p: dict) -> UwoParams:
     s.bias_kind = int(p["bias_kind"])
     s.bias_a, s.bias_b, s.bias_c = (float(v) for v in p["bias"])
     s.chunk_extent = int(p["chunk_extent"])
+    for i, v in enumerate(p.get("offset", (0.0, 0.0, 0.0))):
+        s.offset[i] = float(v)
     return s
 
 
@@ -239,3 +242,62 @@ def build_chunks(params: dict, coords: np.ndarray, n: int, nthreads: int = 1, de
     lib().uwo_build_chunks(ctypes.byref(s), _ptr(c, ctypes.c_int32), len(c), n,
                            0 if dedup == "linear" else 1, nthreads, res)
     return np.array([(r.n_verts, r.n_inds, r.n_active, r.early_blank) for r in res], np.int64)
+
+
+def _adjoint_lib():
+    L = lib()
+    if not hasattr(L, "_adjoint_ready"):
+        P = ctypes.POINTER
+        L.uwo_adjoint_chunk.argtypes = [P(UwoParams), P(ctypes.c_int32), ctypes.c_int32, P(ctypes.c_uint8),
+                                        P(ctypes.c_double), ctypes.c_double, P(ctypes.c_double), P(ctypes.c_double),
+                                        P(ctypes.c_double), P(ctypes.c_int32)]
+        L.uwo_adjoint_chunk.restype = ctypes.c_double
+        L.uwo_plane_loss_dens.argtypes = [ctypes.c_int32, ctypes.c_double, ctypes.c_double, P(ctypes.c_double),
+                                          P(ctypes.c_uint8), ctypes.c_double, P(ctypes.c_double)]
+        L.uwo_plane_loss_dens.restype = ctypes.c_double
+        L.uwo_perlin3_grad.argtypes = [P(ctypes.c_uint8), ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                       P(ctypes.c_double)]
+        L.uwo_perlin3_grad.restype = ctypes.c_double
+        L._adjoint_ready = True
+    return L
+
+
+def perlin3_grad(P, x, y, z):
+    g = np.zeros(3, np.float64)
+    P = np.ascontiguousarray(P, np.uint8)
+    v = _adjoint_lib().uwo_perlin3_grad(_ptr(P, ctypes.c_uint8), x, y, z, _ptr(g, ctypes.c_double))
+    return v, g
+
+
+def adjoint_chunk(params: dict, coord, n: int, target_z: float, cases=None, grads: bool = True, rho=None):
+    """FP64 plane loss of one chunk, sum over vertices of (v_z_local - target_z)^2, and its analytic
+    gradients w.r.t. the densities, octave amplitudes and the noise-space offset. `cases` freezes
+    the topology (n^3 u8); None classifies the FP64 densities."""
+    L = _adjoint_lib()
+    s = make_params(params)
+    c = np.ascontiguousarray(coord, np.int32)
+    S = (n + 1) ** 3
+    gd = np.zeros(S, np.float64)
+    ga = np.zeros(params["octaves"], np.float64)
+    go = np.zeros(3, np.float64)
+    nv = ctypes.c_int32()
+    cs = None if cases is None else np.ascontiguousarray(cases, np.uint8)
+    rr = None if rho is None else np.ascontiguousarray(rho, np.float64)
+    loss = L.uwo_adjoint_chunk(ctypes.byref(s), _ptr(c, ctypes.c_int32), n,
+                               _ptr(cs, ctypes.c_uint8) if cs is not None else None,
+                               _ptr(rr, ctypes.c_double) if rr is not None else None, target_z,
+                               _ptr(gd, ctypes.c_double) if grads else None,
+                               _ptr(ga, ctypes.c_double) if grads else None,
+                               _ptr(go, ctypes.c_double) if grads else None, ctypes.byref(nv))
+    return dict(loss=loss, nverts=nv.value, g_dens=gd, g_amp=ga, g_off=go)
+
+
+def plane_loss_dens(n: int, step: float, iso: float, rho64: np.ndarray, cases: np.ndarray, target_z: float,
+                    grads: bool = False):
+    L = _adjoint_lib()
+    r = np.ascontiguousarray(rho64, np.float64)
+    cs = np.ascontiguousarray(cases, np.uint8)
+    gd = np.zeros(r.size, np.float64)
+    loss = L.uwo_plane_loss_dens(n, step, iso, _ptr(r, ctypes.c_double), _ptr(cs, ctypes.c_uint8), target_z,
+                                 _ptr(gd, ctypes.c_double) if grads else None)
+    return (loss, gd) if grads else loss

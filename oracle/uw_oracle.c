@@ -69,6 +69,7 @@ typedef struct uwo_params {
     float bias_b;       /* terrace: divisor (32.0);    linear: offset                  */
     float bias_c;       /* terrace: modulus (0.25)                                     */
     int32_t chunk_extent; /* world units per chunk edge (chunk.rs:5)                   */
+    double offset[3];     /* world-space shift of every sample (0 = reference)         */
 } uwo_params;
 
 /* ------------------------------------------------------------------------------------- */
@@ -245,11 +246,13 @@ double uwo_iso_at_fp64(const uint8_t* P, const uwo_params* prm, double x, double
  * exact, so this equals the reference's division by CHUNK_SIZE bit for bit. */
 float uwo_step(const uwo_params* prm, int32_t n) { return (float)prm->chunk_extent / (float)n; }
 
-double uwo_coord(const uwo_params* prm, int32_t c, int32_t n, int32_t i) {
+double uwo_coord_axis(const uwo_params* prm, int axis, int32_t c, int32_t n, int32_t i) {
     float step = uwo_step(prm, n);
     double local = (double)i * (double)step;
-    return (local + (double)(prm->chunk_extent * c)) * prm->frequency;
+    return ((local + (double)(prm->chunk_extent * c)) + prm->offset[axis]) * prm->frequency;
 }
+
+double uwo_coord(const uwo_params* prm, int32_t c, int32_t n, int32_t i) { return uwo_coord_axis(prm, 0, c, n, i); }
 
 /* Density grid of one chunk, (n+1)^3 samples, index x (n+1)^2 + y (n+1) + z. */
 void uwo_density_ref(const uwo_params* prm, const int32_t* coord, int32_t n, float* out) {
@@ -257,11 +260,11 @@ void uwo_density_ref(const uwo_params* prm, const int32_t* coord, int32_t n, flo
     uwo_perm(prm->seed, P);
     int32_t L = n + 1;
     for (int32_t x = 0; x < L; ++x) {
-        double px = uwo_coord(prm, coord[0], n, x);
+        double px = uwo_coord_axis(prm, 0, coord[0], n, x);
         for (int32_t y = 0; y < L; ++y) {
-            double py = uwo_coord(prm, coord[1], n, y);
+            double py = uwo_coord_axis(prm, 1, coord[1], n, y);
             for (int32_t z = 0; z < L; ++z) {
-                double pz = uwo_coord(prm, coord[2], n, z);
+                double pz = uwo_coord_axis(prm, 2, coord[2], n, z);
                 out[((int64_t)x * L + y) * L + z] = uwo_iso_at(P, prm, px, py, pz);
             }
         }
@@ -273,11 +276,11 @@ void uwo_density_fp64(const uwo_params* prm, const int32_t* coord, int32_t n, do
     uwo_perm(prm->seed, P);
     int32_t L = n + 1;
     for (int32_t x = 0; x < L; ++x) {
-        double px = uwo_coord(prm, coord[0], n, x);
+        double px = uwo_coord_axis(prm, 0, coord[0], n, x);
         for (int32_t y = 0; y < L; ++y) {
-            double py = uwo_coord(prm, coord[1], n, y);
+            double py = uwo_coord_axis(prm, 1, coord[1], n, y);
             for (int32_t z = 0; z < L; ++z) {
-                double pz = uwo_coord(prm, coord[2], n, z);
+                double pz = uwo_coord_axis(prm, 2, coord[2], n, z);
                 out[((int64_t)x * L + y) * L + z] = uwo_iso_at_fp64(P, prm, px, py, pz);
             }
         }
@@ -572,4 +575,214 @@ int uwo_build_chunks(const uwo_params* prm, const int32_t* coords, int64_t nchun
     }
     for (int t = 0; t < started; ++t) pthread_join(th[t], NULL);
     return 0;
+}
+
+/* ------------------------------------------------------------------------------------- */
+/* Adjoint oracle (FP64). The reference has no differentiation; this is the analytic gradient
+ * the engine's adjoint kernel is checked against, plus the loss itself for finite differences.
+ *
+ * Loss of one chunk: L = sum over the chunk's vertices v (first-seen records, as emitted) of
+ * (v_z - origin_z - target_z)^2. With the cube cases held fixed, L is smooth in the densities:
+ *   v = s_a + t (s_b - s_a),  t = (iso - rho_a) / (rho_b - rho_a)
+ *   dt/drho_a = (iso - rho_b) / D^2,  dt/drho_b = (rho_a - iso) / D^2,  D = rho_b - rho_a.
+ * Densities depend on the octave amplitudes A_j and a world-space offset delta of every sample:
+ *   rho = bias(p_z) + sum_k A_k n_k(f_k p) / sum_k A_k,
+ *   drho/dA_j = (n_j - fbm) / sum A,
+ *   drho/ddelta = freq (sum_k A_k f_k grad n_k / sum A + bias'(p_z) e_z)   (delta in world units),
+ * with bias' = slope for the linear bias and 0 (almost everywhere) for the terrace. A clamped
+ * octave (|n| = 1) has zero gradient. */
+
+/* perlin_3d value and its gradient with respect to the point. */
+double uwo_perlin3_grad(const uint8_t* P, double px, double py, double pz, double* grad) {
+    const double SCALE_FACTOR = 1.1547005383792515;
+    static const int8_t G[16][3] = {{1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1},
+                                    {1, 0, -1}, {-1, 0, -1}, {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1},
+                                    {1, 1, 0},  {-1, 1, 0}, {0, -1, 1}, {0, -1, -1}};
+    double fx = floor(px), fy = floor(py), fz = floor(pz);
+    int64_t cx = (int64_t)fx, cy = (int64_t)fy, cz = (int64_t)fz;
+    double d[3] = {px - fx, py - fy, pz - fz};
+    double g[8], gg[8][3];
+    for (int c = 0; c < 8; ++c) {
+        int ox = c & 1, oy = (c >> 1) & 1, oz = (c >> 2) & 1;
+        uint32_t h = uwo_hash3(P, cx + ox, cy + oy, cz + oz) & 15u;
+        double v[3] = {d[0] - ox, d[1] - oy, d[2] - oz};
+        g[c] = G[h][0] * v[0] + G[h][1] * v[1] + G[h][2] * v[2];
+        for (int k = 0; k < 3; ++k) gg[c][k] = G[h][k];
+    }
+    double s[3], ds[3];
+    for (int k = 0; k < 3; ++k) {
+        double t = d[k];
+        s[k] = uwo_quintic(t);
+        ds[k] = 30.0 * t * t * (t - 1.0) * (t - 1.0);
+    }
+    /* trilinear interpolation with weights w_c = prod_k (o_k ? s_k : 1 - s_k) */
+    double r = 0.0, dr[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 8; ++c) {
+        int o[3] = {c & 1, (c >> 1) & 1, (c >> 2) & 1};
+        double w[3], dw[3];
+        for (int k = 0; k < 3; ++k) {
+            w[k] = o[k] ? s[k] : 1.0 - s[k];
+            dw[k] = o[k] ? ds[k] : -ds[k];
+        }
+        double W = w[0] * w[1] * w[2];
+        r += W * g[c];
+        dr[0] += dw[0] * w[1] * w[2] * g[c] + W * gg[c][0];
+        dr[1] += w[0] * dw[1] * w[2] * g[c] + W * gg[c][1];
+        dr[2] += w[0] * w[1] * dw[2] * g[c] + W * gg[c][2];
+    }
+    r *= SCALE_FACTOR;
+    if (r < -1.0 || r > 1.0) {
+        grad[0] = grad[1] = grad[2] = 0.0;
+        return r < -1.0 ? -1.0 : 1.0;
+    }
+    for (int k = 0; k < 3; ++k) grad[k] = dr[k] * SCALE_FACTOR;
+    return r;
+}
+
+/* Per-sample density derivatives: drho/dA (octaves values) and drho/ddelta (3). */
+static void uwo_density_derivs(const uint8_t* P, const uwo_params* prm, double x, double y, double z, double* dA,
+                               double* dD) {
+    double nk[UWO_MAX_OCTAVES], total = 0.0, m = 0.0, freq = 1.0, gsum[3] = {0, 0, 0};
+    for (int k = 0; k < prm->octaves; ++k) {
+        double g[3];
+        nk[k] = uwo_perlin3_grad(P, x * freq, y * freq, z * freq, g);
+        total += nk[k] * prm->amplitudes[k];
+        m += prm->amplitudes[k];
+        for (int i = 0; i < 3; ++i) gsum[i] += prm->amplitudes[k] * freq * g[i];
+        freq *= prm->lacunarity;
+    }
+    double fbm = total / m;
+    for (int k = 0; k < prm->octaves; ++k) dA[k] = (nk[k] - fbm) / m;
+    for (int i = 0; i < 3; ++i) dD[i] = gsum[i] / m;
+    if (prm->bias_kind == 1) dD[2] += (double)prm->bias_a;
+    for (int i = 0; i < 3; ++i) dD[i] *= prm->frequency; /* the offset is in world units */
+}
+
+/* FP64 plane loss of one chunk and (optionally) its gradients.
+ * cases_in: NULL to classify the FP64 densities, else the frozen n^3 cube cases.
+ * rho_in: NULL to evaluate the FP64 densities, else given densities (S doubles) for the lerps;
+ * the amplitude/offset chain rule always evaluates the FP64 field derivatives.
+ * g_dens (S doubles), g_amp (octaves), g_off (3): optional outputs.
+ * Returns the loss; *nverts receives the number of vertices. */
+double uwo_adjoint_chunk(const uwo_params* prm, const int32_t* coord, int32_t n, const uint8_t* cases_in,
+                         const double* rho_in, double target_z, double* g_dens, double* g_amp, double* g_off,
+                         int32_t* nverts) {
+    uint8_t P[256];
+    uwo_perm(prm->seed, P);
+    const int32_t L = n + 1;
+    const int64_t S = (int64_t)L * L * L;
+    double* rho = (double*)malloc((size_t)S * sizeof(double));
+    double* g = (double*)calloc((size_t)S, sizeof(double));
+    uint8_t* seen_key = NULL;
+    if (rho_in) memcpy(rho, rho_in, (size_t)S * sizeof(double));
+    else uwo_density_fp64(prm, coord, n, rho);
+    const double iso = (double)prm->iso;
+    const double step = (double)uwo_step(prm, n);
+    uwo_map map;
+    uwo_map_init(&map, (uint64_t)5 * n * L * L + 16);
+    double loss = 0.0;
+    int64_t nv = 0;
+    for (int32_t x = 0; x < n; ++x)
+        for (int32_t y = 0; y < n; ++y)
+            for (int32_t z = 0; z < n; ++z) {
+                int32_t cc[8][3];
+                for (int i = 0; i < 8; ++i) {
+                    uint8_t o = UW_MC_CORNER_OFS[i];
+                    cc[i][0] = x + (o & 1);
+                    cc[i][1] = y + ((o >> 1) & 1);
+                    cc[i][2] = z + ((o >> 2) & 1);
+                }
+                uint32_t cs = 0;
+                int64_t cell = ((int64_t)x * n + y) * n + z;
+                if (cases_in) cs = cases_in[cell];
+                else
+                    for (int i = 0; i < 8; ++i)
+                        if (rho[((int64_t)cc[i][0] * L + cc[i][1]) * L + cc[i][2]] < iso) cs |= 1u << i;
+                uint64_t row = UW_MC_TRI_PACKED[cs];
+                int ntri = (int)(row & 15u);
+                for (int j = 0; j < 3 * ntri; ++j) {
+                    int e = (int)((row >> (4 + 4 * j)) & 15u);
+                    const int32_t* ca = cc[UW_MC_EDGE_A_IDX[e]];
+                    const int32_t* cb = cc[UW_MC_EDGE_B_IDX[e]];
+                    int64_t ia = ((int64_t)ca[0] * L + ca[1]) * L + ca[2];
+                    int64_t ib = ((int64_t)cb[0] * L + cb[1]) * L + cb[2];
+                    int found;
+                    uint32_t* slot = uwo_map_slot(&map, (uint64_t)ia * (uint64_t)S + (uint64_t)ib, &found);
+                    if (found) continue;
+                    *slot = (uint32_t)nv++;
+                    double ra = rho[ia], rb = rho[ib], D = rb - ra;
+                    double t = (iso - ra) / D;
+                    double za = ca[2] * step, zb = cb[2] * step;
+                    double r = (za + t * (zb - za)) - target_z;
+                    loss += r * r;
+                    double dz = 2.0 * r * (zb - za);
+                    g[ia] += dz * (iso - rb) / (D * D);
+                    g[ib] += dz * (ra - iso) / (D * D);
+                }
+            }
+    uwo_map_free(&map);
+    free(seen_key);
+    if (g_dens) memcpy(g_dens, g, (size_t)S * sizeof(double));
+    if (g_amp || g_off) {
+        double ga[UWO_MAX_OCTAVES] = {0}, go[3] = {0, 0, 0};
+        double dA[UWO_MAX_OCTAVES], dD[3];
+        for (int32_t x = 0; x < L; ++x)
+            for (int32_t y = 0; y < L; ++y)
+                for (int32_t z = 0; z < L; ++z) {
+                    int64_t s = ((int64_t)x * L + y) * L + z;
+                    if (g[s] == 0.0) continue;
+                    uwo_density_derivs(P, prm, uwo_coord_axis(prm, 0, coord[0], n, x), uwo_coord_axis(prm, 1, coord[1], n, y),
+                                       uwo_coord_axis(prm, 2, coord[2], n, z), dA, dD);
+                    for (int k = 0; k < prm->octaves; ++k) ga[k] += g[s] * dA[k];
+                    for (int i = 0; i < 3; ++i) go[i] += g[s] * dD[i];
+                }
+        if (g_amp) for (int k = 0; k < prm->octaves; ++k) g_amp[k] = ga[k];
+        if (g_off) for (int i = 0; i < 3; ++i) g_off[i] = go[i];
+    }
+    free(rho);
+    free(g);
+    if (nverts) *nverts = (int32_t)nv;
+    return loss;
+}
+
+/* Plane loss and its density gradient from given FP64 densities and frozen cube cases (the mesh
+ * topology). Used for finite differences on single density samples. */
+double uwo_plane_loss_dens(int32_t n, double step, double iso, const double* rho, const uint8_t* cases,
+                           double target_z, double* g_dens) {
+    const int32_t L = n + 1;
+    const int64_t S = (int64_t)L * L * L;
+    if (g_dens) memset(g_dens, 0, (size_t)S * sizeof(double));
+    uwo_map map;
+    uwo_map_init(&map, (uint64_t)5 * n * L * L + 16);
+    double loss = 0.0;
+    int64_t nv = 0;
+    for (int32_t x = 0; x < n; ++x)
+        for (int32_t y = 0; y < n; ++y)
+            for (int32_t z = 0; z < n; ++z) {
+                uint32_t cs = cases[((int64_t)x * n + y) * n + z];
+                uint64_t row = UW_MC_TRI_PACKED[cs];
+                int ntri = (int)(row & 15u);
+                for (int j = 0; j < 3 * ntri; ++j) {
+                    int e = (int)((row >> (4 + 4 * j)) & 15u);
+                    uint8_t oa = UW_MC_CORNER_OFS[UW_MC_EDGE_A_IDX[e]], ob = UW_MC_CORNER_OFS[UW_MC_EDGE_B_IDX[e]];
+                    int64_t ia = ((int64_t)(x + (oa & 1)) * L + (y + ((oa >> 1) & 1))) * L + (z + ((oa >> 2) & 1));
+                    int64_t ib = ((int64_t)(x + (ob & 1)) * L + (y + ((ob >> 1) & 1))) * L + (z + ((ob >> 2) & 1));
+                    int found;
+                    uint32_t* slot = uwo_map_slot(&map, (uint64_t)ia * (uint64_t)S + (uint64_t)ib, &found);
+                    if (found) continue;
+                    *slot = (uint32_t)nv++;
+                    double ra = rho[ia], rb = rho[ib], D = rb - ra;
+                    double t = (iso - ra) / D;
+                    double za = (z + ((oa >> 2) & 1)) * step, zb = (z + ((ob >> 2) & 1)) * step;
+                    double r = (za + t * (zb - za)) - target_z;
+                    loss += r * r;
+                    if (g_dens) {
+                        double dz = 2.0 * r * (zb - za);
+                        g_dens[ia] += dz * (iso - rb) / (D * D);
+                        g_dens[ib] += dz * (ra - iso) / (D * D);
+                    }
+                }
+            }
+    uwo_map_free(&map);
+    return loss;
 }

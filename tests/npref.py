@@ -75,15 +75,16 @@ def perlin(P: np.ndarray, pts: np.ndarray) -> np.ndarray:
     return np.clip(r * 1.1547005383792515, -1.0, 1.0)
 
 
-def axis_coords(p: dict, c: int, n: int) -> np.ndarray:
+def axis_coords(p: dict, c: int, n: int, axis: int = 0) -> np.ndarray:
     step = np.float32(p["chunk_extent"]) / np.float32(n)
     i = np.arange(n + 1, dtype=np.float64)
-    return (i * np.float64(step) + np.float64(p["chunk_extent"] * c)) * p["frequency"]
+    off = float(p.get("offset", (0.0, 0.0, 0.0))[axis])
+    return ((i * np.float64(step) + np.float64(p["chunk_extent"] * c)) + off) * p["frequency"]
 
 
 def density(p: dict, coord, n: int) -> np.ndarray:
     P = perm(p["seed"])
-    ax = [axis_coords(p, coord[k], n) for k in range(3)]
+    ax = [axis_coords(p, coord[k], n, k) for k in range(3)]
     X, Y, Z = np.meshgrid(ax[0], ax[1], ax[2], indexing="ij")
     pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
     total = np.zeros(len(pts))

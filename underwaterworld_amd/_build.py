@@ -34,6 +34,24 @@ def sources():
     return sorted(glob.glob(os.path.join(CSRC, "*.hip")))
 
 
+ROOT = os.path.dirname(HERE)
+FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_main.cpp")
+FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "facade_main")
+
+
+def build_facade_test(verbose: bool = False) -> str:
+    """The C++ facade driver used by tests/test_gpu_facade.py (links libuwmesh.so by rpath)."""
+    cmd = [HIPCC, "-O2", "-std=c++17", "-o", FACADE_BIN, FACADE_SRC, "-L" + HERE, "-luwmesh",
+           "-Wl,-rpath,$ORIGIN/../../underwaterworld_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("hipcc failed building the C++ facade driver")
+    return FACADE_BIN
+
+
 def build(verbose: bool = False) -> str:
     srcs = sources()
     tmp = LIB + ".tmp"
@@ -47,6 +65,8 @@ def build(verbose: bool = False) -> str:
     if verbose and r.stderr.strip():
         sys.stderr.write(r.stderr)
     os.replace(tmp, LIB)
+    if os.path.exists(FACADE_SRC):
+        build_facade_test(verbose)
     return LIB
 
 

@@ -42,6 +42,7 @@ class UwParams(ctypes.Structure):
         ("bias_b", ctypes.c_float),
         ("bias_c", ctypes.c_float),
         ("chunk_extent", ctypes.c_int32),
+        ("offset", ctypes.c_double * 3),
     ]
 
 
@@ -93,6 +94,11 @@ SIGNATURES = {
     "uw_batched_graph_create": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut),
                                                P(c_void_p)]),
     "uw_graph_launch": (ctypes.c_int, [c_void_p, c_void_p]),
+    "uw_ctx_set_field": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "uw_build_chunks_batched_tape": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut),
+                                                    c_void_p, c_void_p]),
+    "uw_adjoint_backward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p,
+                                           c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p, c_void_p]),
     "uw_graph_destroy": (None, [c_void_p]),
 }
 
@@ -128,6 +134,8 @@ def make_params(p: dict) -> UwParams:
     s.bias_kind = int(p["bias_kind"])
     s.bias_a, s.bias_b, s.bias_c = (float(v) for v in p["bias"])
     s.chunk_extent = int(p["chunk_extent"])
+    for i, v in enumerate(p.get("offset", (0.0, 0.0, 0.0))):
+        s.offset[i] = float(v)
     return s
 
 

@@ -1,0 +1,128 @@
+"""Autotuner for the batched persistent kernel's launch shape.
+
+    python -m underwaterworld_amd.autotune [--quick] [--out underwaterworld_amd/tuning.json]
+
+For each workload (the 4096-chunk 12^3 grid with 4 octaves, the 8192-chunk mixed-resolution set
+with 6 octaves) it times, on the device, every combination of
+    block   threads per workgroup: 128, 256, 512
+    budget  LDS bytes allowed for a chunk's density slab ring: a chunk whose (n+1)^3 densities fit
+            is built in one slab (one density evaluation); larger chunks use the largest x-slab
+            that fits (densities evaluated twice)
+    grid    persistent workgroups: all that fit (0), or a fixed multiple of the CU count
+and writes the fastest shape per octave count to tuning.json, which `Perlin` applies when it
+creates a context. Every candidate's mesh is checked against the first candidate's (all shapes
+must produce identical meshes) before its time is accepted.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "tuning.json")
+
+BLOCKS = (128, 256, 512)
+BUDGETS = (6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024, 40 * 1024, 56 * 1024)
+GRIDS = (0,)
+
+
+def slabs_for_budget(budget: int):
+    s = [1]
+    for n in range(1, 33):
+        L = n + 1
+        if 4 * L ** 3 <= budget:
+            s.append(n)
+        else:
+            s.append(max(1, min(n, budget // (4 * L * L) - 1)))
+    return s
+
+
+def _mesh_digest(batch, nchunks):
+    info = batch.info()
+    h = []
+    for i in range(nchunks):
+        v, ix = batch.chunk_mesh(i, info)
+        h.append((int(info[i]["n_verts"]), int(info[i]["n_inds"]), hash(v.tobytes()), hash(ix.tobytes())))
+    return h
+
+
+def tune_workload(perlin, coords, res, steps=10, quick=False, log=print):
+    import torch
+
+    from .engine import BatchedGraph, get_batch_tuning, set_batch_tuning
+
+    base = get_batch_tuning(perlin)
+    results = []
+    ref_digest = None
+    blocks = (256,) if quick else BLOCKS
+    budgets = (10 * 1024, 20 * 1024) if quick else BUDGETS
+    stream = torch.cuda.current_stream()
+    for block in blocks:
+        for budget in budgets:
+            for grid in GRIDS:
+                t = dict(base, block=block, grid=grid, slab=slabs_for_budget(budget))
+                try:
+                    set_batch_tuning(perlin, t)
+                    g = BatchedGraph(perlin, coords, res)
+                except Exception as e:  # shape does not fit (LDS) -> skip
+                    log(f"  block={block} budget={budget} grid={grid}: skipped ({e})")
+                    continue
+                g.launch(stream.cuda_stream)
+                torch.cuda.synchronize()
+                dig = _mesh_digest(g.batch, len(coords))
+                if ref_digest is None:
+                    ref_digest = dig
+                elif dig != ref_digest:
+                    raise RuntimeError(f"tuning block={block} budget={budget} changed the mesh")
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                for _ in range(steps):
+                    g.launch(stream.cuda_stream)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                ms = e0.elapsed_time(e1) / steps
+                g.close()
+                results.append((ms, t))
+                log(f"  block={block} budget={budget} grid={grid}: {ms:.4f} ms")
+    set_batch_tuning(perlin, base)
+    results.sort(key=lambda r: r[0])
+    return results
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--quick", action="store_true")
+    ap.add_argument("--steps", type=int, default=10)
+    args = ap.parse_args()
+    sys.path.insert(0, os.path.dirname(HERE))
+    from . import params
+    from .engine import Perlin
+
+    report = {"by_octaves": {}, "timings": {}}
+    jobs = [("chunks12", params.grid_coords(-8, 8), [12] * 4096)]
+    mc, mr = params.mixed_workload(8192, 7)
+    jobs.append(("mixed", mc, mr))
+    for name, coords, res in jobs:
+        p = params.preset(name)
+        perlin = Perlin(p, apply_tuning=False)
+        print(f"tuning {name} ({len(coords)} chunks, {p['octaves']} octaves)", flush=True)
+        r = tune_workload(perlin, np.asarray(coords, np.int32), res, steps=args.steps, quick=args.quick)
+        best_ms, best = r[0]
+        report["by_octaves"][str(p["octaves"])] = best
+        report["timings"][name] = [{"ms": round(ms, 5), "block": t["block"], "grid": t["grid"],
+                                    "slab12": t["slab"][12], "slab24": t["slab"][24], "slab32": t["slab"][32]}
+                                   for ms, t in r]
+        print(f"best {name}: {best_ms:.4f} ms block={best['block']} slab12={best['slab'][12]} "
+              f"slab24={best['slab'][24]} slab32={best['slab'][32]}", flush=True)
+    with open(args.out, "w") as f:
+        json.dump(report, f, indent=1)
+    print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

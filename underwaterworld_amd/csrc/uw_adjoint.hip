@@ -1,0 +1,265 @@
+// Adjoint of the chunk builder: gradients of a loss on the emitted vertex positions with respect to
+// the density samples, the octave amplitudes and a world-space offset of every sample coordinate.
+// The reference has no differentiation; this follows the forward defined by
+//   v = s_a + t (s_b - s_a),  t = (iso - rho_a) / (rho_b - rho_a)       (chunk.rs:178-213)
+//   rho = bias(p_z) + sum_k A_k n_k(f_k p) / sum_k A_k                    (perlin_util.rs:6-29)
+// with the cube cases held fixed (the mesh is piecewise smooth in its inputs):
+//   dv/drho_a = (s_b - s_a) (iso - rho_b) / D^2,   dv/drho_b = (s_b - s_a) (rho_a - iso) / D^2
+//   drho/dA_j = (n_j - fbm) / sum A
+//   drho/ddelta = freq (sum_k A_k f_k grad n_k / sum A + bias'(p_z) e_z)
+// Only the forward's vertex tape is kept: the octave noise values and gradients at each vertex's
+// two endpoint samples are recomputed here (per-axis lattice data of the chunk in LDS, then eight
+// hashed gradients, fades and fade derivatives per octave), so a large volume needs no stored
+// density or per-octave fields.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <cmath>
+
+#include "uw_mesh.h"
+#include "uw_noise.h"
+
+namespace {
+
+constexpr int kMaxL = UW_BATCH_MAX_RESOLUTION + 1;
+constexpr int kBlock = 256;
+
+// Octave value and gradient with respect to the octave coordinate (f32), given the per-axis
+// lattice data and fade derivatives s'(d) = 30 d^2 (d - 1)^2.
+__device__ __forceinline__ float uw_octave_grad(const uw_axis& X, const uw_axis& Y, const uw_axis& Z, float dfx,
+                                                float dfy, float dfz, const uint8_t* P, const float4* GP,
+                                                float g[3]) {
+    const uint32_t a0 = X.bits & 255, a1 = X.bits >> 8;
+    const uint32_t y0 = Y.bits & 255, y1 = Y.bits >> 8;
+    const uint32_t z0 = Z.bits & 255, z1 = Z.bits >> 8;
+    const uint32_t b00 = P[a0 ^ y0], b01 = P[a0 ^ y1], b10 = P[a1 ^ y0], b11 = P[a1 ^ y1];
+    const float4 G000 = GP[b00 ^ z0], G100 = GP[b10 ^ z0], G010 = GP[b01 ^ z0], G110 = GP[b11 ^ z0];
+    const float4 G001 = GP[b00 ^ z1], G101 = GP[b10 ^ z1], G011 = GP[b01 ^ z1], G111 = GP[b11 ^ z1];
+    const float g000 = uw_gdot(G000, X.d, Y.d, Z.d), g100 = uw_gdot(G100, X.dm1, Y.d, Z.d);
+    const float g010 = uw_gdot(G010, X.d, Y.dm1, Z.d), g110 = uw_gdot(G110, X.dm1, Y.dm1, Z.d);
+    const float g001 = uw_gdot(G001, X.d, Y.d, Z.dm1), g101 = uw_gdot(G101, X.dm1, Y.d, Z.dm1);
+    const float g011 = uw_gdot(G011, X.d, Y.dm1, Z.dm1), g111 = uw_gdot(G111, X.dm1, Y.dm1, Z.dm1);
+    const float a = X.fade, b = Y.fade, c = Z.fade;
+    const float x00 = uw_lerp(a, g000, g100), x10 = uw_lerp(a, g010, g110);
+    const float x01 = uw_lerp(a, g001, g101), x11 = uw_lerp(a, g011, g111);
+    const float yv0 = uw_lerp(b, x00, x10), yv1 = uw_lerp(b, x01, x11);
+    float r = uw_lerp(c, yv0, yv1) * UW_PERLIN_SCALE;
+    // d/dx
+    const float dx00 = fmaf(dfx, g100 - g000, uw_lerp(a, G000.x, G100.x));
+    const float dx10 = fmaf(dfx, g110 - g010, uw_lerp(a, G010.x, G110.x));
+    const float dx01 = fmaf(dfx, g101 - g001, uw_lerp(a, G001.x, G101.x));
+    const float dx11 = fmaf(dfx, g111 - g011, uw_lerp(a, G011.x, G111.x));
+    const float drx = uw_lerp(c, uw_lerp(b, dx00, dx10), uw_lerp(b, dx01, dx11));
+    // d/dy
+    const float dy0 = fmaf(dfy, x10 - x00, uw_lerp(b, uw_lerp(a, G000.y, G100.y), uw_lerp(a, G010.y, G110.y)));
+    const float dy1 = fmaf(dfy, x11 - x01, uw_lerp(b, uw_lerp(a, G001.y, G101.y), uw_lerp(a, G011.y, G111.y)));
+    const float dry = uw_lerp(c, dy0, dy1);
+    // d/dz
+    const float ez0 = uw_lerp(b, uw_lerp(a, G000.z, G100.z), uw_lerp(a, G010.z, G110.z));
+    const float ez1 = uw_lerp(b, uw_lerp(a, G001.z, G101.z), uw_lerp(a, G011.z, G111.z));
+    const float drz = fmaf(dfz, yv1 - yv0, uw_lerp(c, ez0, ez1));
+    if (r > 1.0f || r < -1.0f) {
+        g[0] = g[1] = g[2] = 0.0f;
+        return r > 1.0f ? 1.0f : -1.0f;
+    }
+    g[0] = drx * UW_PERLIN_SCALE;
+    g[1] = dry * UW_PERLIN_SCALE;
+    g[2] = drz * UW_PERLIN_SCALE;
+    return r;
+}
+
+struct BackArgs {
+    uw_field f;
+    const uw_tables* tab;
+    const int32_t* coords;
+    const int32_t* res;
+    const uw_chunk_info* info;
+    const uw_vertex_tape* tape;
+    const float* grad_verts;
+    float* grad_dens;
+    int64_t dens_stride;
+    double* grad_amp;
+    double* grad_off;
+};
+
+template <int OMAX>
+__global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
+    __shared__ float4 GP[256];
+    __shared__ uint8_t P[256];
+    __shared__ uw_axis AX[OMAX * kMaxL], AY[OMAX * kMaxL], AZ[OMAX * kMaxL];
+    __shared__ float DF[3][OMAX * kMaxL];
+    __shared__ float ADJ[kMaxL], POST[kMaxL];
+    __shared__ double red[kBlock / 64][OMAX + 3];
+
+    const int tid = threadIdx.x;
+    const int32_t c = blockIdx.x;
+    const int32_t n = a.res[c];
+    const int32_t O = a.f.octaves;
+    const uw_chunk_info ci = a.info[c];
+    const bool skip = n < 1 || n > UW_BATCH_MAX_RESOLUTION || (ci.flags & (UW_CHUNK_OVERFLOW | UW_CHUNK_INVALID)) ||
+                      ci.n_verts == 0;
+    const bool want_params = a.grad_amp != nullptr || a.grad_off != nullptr;
+    if (skip) {
+        if (tid < O && a.grad_amp) a.grad_amp[int64_t(c) * O + tid] = 0.0;
+        if (tid < 3 && a.grad_off) a.grad_off[int64_t(c) * 3 + tid] = 0.0;
+        return;
+    }
+    const int32_t L = n + 1;
+    const float step = float(a.f.extent) / float(n);
+    const int32_t cx = a.coords[3 * c], cy = a.coords[3 * c + 1], cz = a.coords[3 * c + 2];
+    for (int i = tid; i < 256; i += kBlock) {
+        const uint8_t p = a.tab->perm[i];
+        P[i] = p;
+        GP[i] = a.tab->grad[p & 15];
+    }
+    __syncthreads();
+    const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
+    if (want_params) {
+        uw_build_chunk_tables(a.f, P, cx, cy, cz, L, step, T, tid, kBlock);
+        const int32_t cc[3] = {cx, cy, cz};
+        for (int e = tid; e < 3 * O * L; e += kBlock) {
+            const int axis = e / (O * L), r = e - axis * O * L;
+            const int k = r / L, i = r - k * L;
+            const double pk = uw_sample_coord(a.f, axis, cc[axis], i, step) * a.f.octave_freq[k];
+            const double d = pk - floor(pk);
+            DF[axis][r] = float(30.0 * d * d * (d - 1.0) * (d - 1.0));
+        }
+    }
+    __syncthreads();
+
+    const float iso = a.f.iso;
+    const double inv_sum = 1.0 / double(a.f.amp_sum);
+    const double slope = a.f.bias_kind == UW_BIAS_LINEAR ? double(a.f.bias_a) : 0.0;
+    double acc[OMAX + 3];
+#pragma unroll
+    for (int k = 0; k < OMAX + 3; ++k) acc[k] = 0.0;
+    float* gd = a.grad_dens ? a.grad_dens + int64_t(c) * a.dens_stride : nullptr;
+    const int32_t LL = L * L;
+    for (uint32_t i = tid; i < ci.n_verts; i += kBlock) {
+        const uint64_t v = ci.vert_off + i;
+        const uw_vertex_tape tv = a.tape[v];
+        const int32_t diff = int32_t(tv.b) - int32_t(tv.a);
+        const int axis = (diff == 1 || diff == -1) ? 2 : ((diff == L || diff == -L) ? 1 : 0);
+        const float dir = diff > 0 ? step : -step;
+        const float gv = a.grad_verts[3 * v + axis] * dir;
+        const float D = tv.rho_b - tv.rho_a;
+        const float inv_d2 = 1.0f / (D * D);
+        const float w[2] = {gv * (iso - tv.rho_b) * inv_d2, gv * (tv.rho_a - iso) * inv_d2};
+        const uint32_t s[2] = {tv.a, tv.b};
+        if (gd) {
+            unsafeAtomicAdd(gd + s[0], w[0]);
+            unsafeAtomicAdd(gd + s[1], w[1]);
+        }
+        if (!want_params) continue;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int32_t x = int32_t(s[e]) / LL, y = (int32_t(s[e]) / L) % L, z = int32_t(s[e]) % L;
+            float nk[OMAX];
+            float total = 0.0f, gx = 0.0f, gy = 0.0f, gz = 0.0f;
+#pragma unroll
+            for (int k = 0; k < OMAX; ++k) {
+                if (k < O) {
+                    float g[3];
+                    nk[k] = uw_octave_grad(AX[k * L + x], AY[k * L + y], AZ[k * L + z], DF[0][k * L + x],
+                                           DF[1][k * L + y], DF[2][k * L + z], P, GP, g);
+                    const float af = a.f.amp[k] * float(a.f.octave_freq[k]);
+                    total = fmaf(a.f.amp[k], nk[k], total);
+                    gx = fmaf(af, g[0], gx);
+                    gy = fmaf(af, g[1], gy);
+                    gz = fmaf(af, g[2], gz);
+                }
+            }
+            const double we = double(w[e]);
+            const double fbm = double(total) * inv_sum;
+#pragma unroll
+            for (int k = 0; k < OMAX; ++k)
+                if (k < O) acc[k] += we * (double(nk[k]) - fbm) * inv_sum;
+            const double fq = a.f.frequency;
+            acc[OMAX + 0] += we * fq * double(gx) * inv_sum;
+            acc[OMAX + 1] += we * fq * double(gy) * inv_sum;
+            acc[OMAX + 2] += we * fq * (double(gz) * inv_sum + slope);
+        }
+    }
+    if (!want_params) return;
+    // block reduction of the per-thread accumulators
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < OMAX + 3; ++k) {
+        double vsum = acc[k];
+        for (int d = 32; d > 0; d >>= 1) vsum += __shfl_down(vsum, d, 64);
+        if (lane == 0) red[wave][k] = vsum;
+    }
+    __syncthreads();
+    if (tid < OMAX + 3) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / 64; ++w) t += red[w][tid];
+        if (tid < OMAX) {
+            if (tid < O && a.grad_amp) a.grad_amp[int64_t(c) * O + tid] = t;
+        } else if (a.grad_off) {
+            a.grad_off[int64_t(c) * 3 + (tid - OMAX)] = t;
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int uw_ctx_set_field(uw_ctx* ctx, const double* amplitudes, const double* offset) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    uw_params p = ctx->params;
+    if (amplitudes) {
+        double s = 0.0;
+        for (int k = 0; k < p.octaves; ++k) {
+            if (!std::isfinite(amplitudes[k])) return uw_fail(ctx, UW_ERR_INVALID_ARG, "amplitudes must be finite");
+            p.amplitudes[k] = amplitudes[k];
+            s += amplitudes[k];
+        }
+        if (s == 0.0) return uw_fail(ctx, UW_ERR_INVALID_ARG, "amplitudes must not sum to zero");
+    }
+    if (offset)
+        for (int i = 0; i < 3; ++i) {
+            if (!std::isfinite(offset[i])) return uw_fail(ctx, UW_ERR_INVALID_ARG, "offset must be finite");
+            p.offset[i] = offset[i];
+        }
+    ctx->params = p;
+    return UW_OK;
+}
+
+int uw_adjoint_backward(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                        const uw_chunk_info* info, const uw_vertex_tape* tape, const float* grad_verts,
+                        float* grad_dens, int64_t dens_stride, double* grad_amp, double* grad_off, void* stream) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    if (nchunks < 0 || (nchunks > 0 && (!coords || !res || !info || !tape || !grad_verts)))
+        return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    const int64_t Lmax = UW_BATCH_MAX_RESOLUTION + 1;
+    if (grad_dens && dens_stride < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "dens_stride must be positive");
+    if (nchunks == 0) return UW_OK;
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    (void)Lmax;
+    BackArgs a;
+    memset(&a, 0, sizeof(a));
+    a.f = uw_make_field(ctx->params);
+    a.tab = ctx->d_tables;
+    a.coords = coords;
+    a.res = res;
+    a.info = info;
+    a.tape = tape;
+    a.grad_verts = grad_verts;
+    a.grad_dens = grad_dens;
+    a.dens_stride = dens_stride;
+    a.grad_amp = grad_amp;
+    a.grad_off = grad_off;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int O = a.f.octaves;
+    if (int rc = uw_prof_start(ctx, st)) return rc;
+    if (O <= 4) hipLaunchKernelGGL(k_backward<4>, dim3(uint32_t(nchunks)), dim3(kBlock), 0, st, a);
+    else if (O <= 8) hipLaunchKernelGGL(k_backward<8>, dim3(uint32_t(nchunks)), dim3(kBlock), 0, st, a);
+    else hipLaunchKernelGGL(k_backward<16>, dim3(uint32_t(nchunks)), dim3(kBlock), 0, st, a);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
+    UW_HIP(ctx, hipGetLastError());
+    return UW_OK;
+}
+
+}  // extern "C"

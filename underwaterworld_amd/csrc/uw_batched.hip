@@ -43,6 +43,7 @@ struct BatchArgs {
     uw_totals* totals;
     float* verts;
     uint32_t* inds;
+    uw_vertex_tape* tape;  // optional per-vertex record for the adjoint
     uint64_t vcap, icap;
     int8_t sx[kMaxN + 1]; // cell planes per slab for each resolution
 };
@@ -326,6 +327,18 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                                     dst[0] = p[0];
                                     dst[1] = p[1];
                                     dst[2] = p[2];
+                                    if (a.tape) {
+                                        const uint32_t ia = uw_edge_a(ed), ib = uw_edge_b(ed);
+                                        const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib);
+                                        uw_vertex_tape tv;
+                                        tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L +
+                                                        z + int32_t((ca >> 2) & 1u));
+                                        tv.b = uint32_t(((x + int32_t(cb & 1u)) * L + y + int32_t((cb >> 1) & 1u)) * L +
+                                                        z + int32_t((cb >> 2) & 1u));
+                                        tv.rho_a = r[ia];
+                                        tv.rho_b = r[ib];
+                                        a.tape[vb + rank] = tv;
+                                    }
                                 }
                             } else {
                                 int32_t qx, qy, qz;
@@ -439,7 +452,8 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
 }
 
 int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, const int32_t* res, int32_t nchunks,
-                   const uw_mesh_out* out, uint32_t* head, uint32_t* order, hipStream_t st) {
+                   const uw_mesh_out* out, uint32_t* head, uint32_t* order, hipStream_t st,
+                   uw_vertex_tape* tape = nullptr) {
     const uw_batch_tuning& tune = ctx->batch_tuning;
     BatchArgs a;
     memset(&a, 0, sizeof(a));
@@ -454,6 +468,7 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.inds = out->inds;
     a.vcap = out->vert_cap;
     a.icap = out->ind_cap;
+    a.tape = tape;
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
     a.order = order;
     a.head = head;
@@ -489,6 +504,23 @@ int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* r
         return UW_OK;
     }
     return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, st);
+}
+
+int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                                 const uw_mesh_out* out, uw_vertex_tape* tape, void* stream) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    if (!args_ok(nchunks, coords, res, out) || (out->vert_cap > 0 && !tape))
+        return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    BatchLaunch bl;
+    if (int rc = prepare_batched(ctx, nchunks > 0 ? nchunks : 1, &bl)) return rc;
+    if (nchunks == 0) {
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, ctx->d_batch_head, out->totals);
+        UW_HIP(ctx, hipGetLastError());
+        return UW_OK;
+    }
+    return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, st, tape);
 }
 
 int uw_get_batch_tuning(const uw_ctx* ctx, uw_batch_tuning* t) {

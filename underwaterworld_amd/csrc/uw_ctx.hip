@@ -122,6 +122,8 @@ bool params_valid(const uw_params* p, std::string* why) {
     }
     if (p->chunk_extent < 1 || p->chunk_extent > (1 << 20)) { *why = "chunk_extent out of range"; return false; }
     if (!isfinite(p->iso)) { *why = "iso must be finite"; return false; }
+    for (int i = 0; i < 3; ++i)
+        if (!isfinite(p->offset[i])) { *why = "offset must be finite"; return false; }
     return true;
 }
 
@@ -131,6 +133,7 @@ uw_field uw_make_field(const uw_params& p) {
     uw_field f;
     memset(&f, 0, sizeof(f));
     f.frequency = p.frequency;
+    for (int i = 0; i < 3; ++i) f.offset[i] = p.offset[i];
     double fk = 1.0, sum = 0.0;
     for (int k = 0; k < p.octaves; ++k) {
         f.octave_freq[k] = fk;

@@ -89,6 +89,7 @@ int uw_hip_fail(uw_ctx* ctx, hipError_t e, const char* what);
 // Noise-side constants derived from uw_params, passed to kernels by value.
 struct uw_field {
     double frequency;
+    double offset[3];                    // world-space shift of every sample coordinate
     double octave_freq[UW_MAX_OCTAVES];  // lacunarity^k by repeated multiplication
     float amp[UW_MAX_OCTAVES];           // amplitudes rounded to f32
     float amp_sum;                       // sum of amplitudes (f64 sum rounded to f32)

@@ -28,11 +28,12 @@ struct __align__(16) uw_axis {
 // 1/(sqrt(3)/2), noise 0.8.2 perlin_3d SCALE_FACTOR, rounded to f32.
 #define UW_PERLIN_SCALE 1.1547005383792515f
 
-// Noise coordinate of sample i of chunk c along one axis (chunk.rs:107-108): all f64, the
-// product by a power-of-two frequency equals the reference's division by CHUNK_SIZE.
-__device__ __forceinline__ double uw_sample_coord(const uw_field& f, int32_t c, int32_t i, float step) {
+// Noise coordinate of sample i of chunk c along `axis` (chunk.rs:107-108): all f64; the product
+// by a power-of-two frequency equals the reference's division by CHUNK_SIZE, and the world-space
+// offset (0 for the reference, where adding it is exact) shifts every sample.
+__device__ __forceinline__ double uw_sample_coord(const uw_field& f, int axis, int32_t c, int32_t i, float step) {
     const double local = double(i) * double(step);
-    return (local + double(f.extent * c)) * f.frequency;
+    return ((local + double(f.extent * c)) + f.offset[axis]) * f.frequency;
 }
 
 __device__ __forceinline__ double uw_quintic64(double t) { return t * t * t * (t * (t * 6.0 - 15.0) + 10.0); }
@@ -169,11 +170,11 @@ __device__ __forceinline__ void uw_build_chunk_tables(const uw_field& f, const u
         const int32_t r = e - axis * O * L;
         const int32_t k = r / L, i = r - k * L;
         const double fk = f.octave_freq[k];
-        if (axis == 0) T.AX[r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, cx, i, step), fk), perm);
-        else if (axis == 1) T.AY[r] = uw_axis_of(uw_sample_coord(f, cy, i, step), fk);
-        else T.AZ[r] = uw_axis_of(uw_sample_coord(f, cz, i, step), fk);
+        if (axis == 0) T.AX[r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, 0, cx, i, step), fk), perm);
+        else if (axis == 1) T.AY[r] = uw_axis_of(uw_sample_coord(f, 1, cy, i, step), fk);
+        else T.AZ[r] = uw_axis_of(uw_sample_coord(f, 2, cz, i, step), fk);
     }
-    for (int i = tid; i < L; i += nthr) uw_bias_terms(f, uw_sample_coord(f, cz, i, step), &T.ADJ[i], &T.POST[i]);
+    for (int i = tid; i < L; i += nthr) uw_bias_terms(f, uw_sample_coord(f, 2, cz, i, step), &T.ADJ[i], &T.POST[i]);
 }
 
 // Densities of x-planes [xa, xb) of a chunk whose tables are in T, one thread per (x, y) line.

@@ -40,7 +40,8 @@ class Perlin:
     """Density-field context: the permutation table of `seed` plus the field parameters, uploaded
     to one GPU (noise::Perlin::new, state.rs:358-359)."""
 
-    def __init__(self, params: dict | None = None, seed: int | None = None, device: int = 0):
+    def __init__(self, params: dict | None = None, seed: int | None = None, device: int = 0,
+                 apply_tuning: bool = True):
         p = dict(params) if params is not None else _params.preset("reference")
         if seed is not None:
             p["seed"] = int(seed)
@@ -51,6 +52,10 @@ class Perlin:
         rc = _lib.lib().uw_ctx_create(ctypes.byref(s), device, ctypes.byref(self._ctx))
         if rc != _lib.UW_OK:
             raise _lib.UwError(rc, "uw_ctx_create failed (parameters or device)")
+        if apply_tuning:
+            t = tuned_shape(int(p["octaves"]))
+            if t is not None:
+                set_batch_tuning(self, t)
 
     @classmethod
     def new(cls, seed: int):
@@ -76,6 +81,20 @@ class Perlin:
             self.close()
         except Exception:
             pass
+
+
+_TUNING = None
+
+
+def tuned_shape(octaves: int):
+    """The autotuner's launch shape for this octave count (underwaterworld_amd/tuning.json)."""
+    global _TUNING
+    if _TUNING is None:
+        import json
+        import os
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
+        _TUNING = json.load(open(path)) if os.path.exists(path) else {}
+    return _TUNING.get("by_octaves", {}).get(str(octaves))
 
 
 def permutation_table(seed: int) -> np.ndarray:
