@@ -56,6 +56,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-resolution workload")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -67,7 +68,8 @@ def main():
     import numpy as np
     import torch
 
-    from underwaterworld_amd import BatchedGraph, Perlin, _lib, params
+    from underwaterworld_amd import BatchedGraph, Perlin, _lib, get_batch_tuning, params, set_batch_tuning
+    from underwaterworld_amd.engine import tuned_shape
     from underwaterworld_amd.engine import _alloc_out, _mesh_out, batch_capacity
 
     L = _lib.lib()
@@ -103,6 +105,10 @@ def main():
     nch = len(coords)
     S = (n + 1) ** 3
     perlin = Perlin(p)
+    if tuned_shape(p["octaves"]) is None:  # no autotuned shape: size the launch for 12^3 chunks
+        t = get_batch_tuning(perlin)
+        t["max_res"] = n
+        set_batch_tuning(perlin, t)
     ct = torch.as_tensor(coords, device=dev)
     rt = torch.full((nch,), n, dtype=torch.int32, device=dev)
     vcap, icap = int(L.uw_max_verts(n)) * nch, int(L.uw_max_inds(n)) * nch
@@ -162,11 +168,42 @@ def main():
         print(f"# mixed workload: {ms_mixed:.4f} ms/step", file=sys.stderr, flush=True)
         g2.close()
 
+    # ---- adjoint: 256 chunks of 24^3, forward with tape + backward (densities, amplitudes, offset) ----
+    adjoint = None
+    if not args.no_adjoint:
+        from underwaterworld_amd.adjoint import ChunkMesher
+        pa = params.preset("adjoint24")
+        ac = [(x, y, z) for x in range(-4, 4) for y in range(-4, 4) for z in range(-3, 1)]
+        pad = Perlin(pa)
+        mesher = ChunkMesher(pad, ac, 24, want_dens_grad=True)
+        mesher.forward()
+        torch.cuda.synchronize()
+        nvt = int(mesher.batch.totals()["n_verts"])
+        gv = torch.zeros((mesher.vert_cap, 3), dtype=torch.float32, device=dev)
+        gv[:nvt, 2] = 1.0
+
+        def adj_step():
+            mesher.forward(stream.cuda_stream)
+            mesher.grad_dens.zero_() if mesher.grad_dens is not None else None
+            mesher.backward(gv, stream.cuda_stream)
+
+        ms_adj = timed(adj_step, max(3, args.steps // 4), 1)
+        L.uw_profile_enable(pad.ctx, 1)
+        mesher.backward(gv, stream.cuda_stream)
+        tot_b, cnt_b = ctypes.c_double(), ctypes.c_uint64()
+        L.uw_profile_read(pad.ctx, ctypes.byref(tot_b), ctypes.byref(cnt_b))
+        L.uw_profile_enable(pad.ctx, 0)
+        adjoint = {"workload": "256 chunks (x,y in [-4,4), z in [-3,1)) x 24^3 cells, 6 octaves, seed 42; "
+                               "forward with vertex tape + backward to 25^3 densities, 6 amplitudes, 3 offsets",
+                   "ms_per_step": round(ms_adj, 4), "grads_per_s": round(len(ac) / (ms_adj / 1e3), 1),
+                   "backward_kernel_ms": round(tot_b.value / max(cnt_b.value, 1), 4), "vertices": nvt}
+        print(f"# adjoint: {ms_adj:.4f} ms/step (forward + backward)", file=sys.stderr, flush=True)
+
     flops = flops_per_sample(p["octaves"]) * S * nch
     achieved = flops / (ms_kernel / 1e3) / 1e12
     roof = {"bound": "fp32-valu", "kernel": "k_batched (persistent: density + mesh)", "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": read_traffic("k_batched_chunks12_pmc.txt"),
+            "traffic": read_traffic("k_batched_chunks12.md"),
             "algorithmic_flop_per_launch": flops, "ms_per_launch": round(ms_kernel, 5),
             "plain_density_kernel": {"ms_per_launch": round(ms_noise_plain, 5),
                                      "achieved_tflops": round(flops / (ms_noise_plain / 1e3) / 1e12, 3)}}
@@ -183,7 +220,7 @@ def main():
 
     headline_ms = ms_graph
     res = {
-        "metric": "chunks/s at 12^3 cells (batched vs plain GPU); voxel samples/s",
+        "metric": "chunks/s at 12^3 cells (batched vs plain GPU); voxel samples/s; adjoint grads/s",
         "value": round(nch / (headline_ms / 1e3), 1),
         "unit": "chunks/s",
         "n_gpus": 1,
@@ -206,6 +243,7 @@ def main():
         "batched_over_plain": round(ms_plain / headline_ms, 3),
         "total_verts": int(gt["n_verts"]), "total_inds": int(gt["n_inds"]), "nonblank": int(gt["n_nonblank"]),
         "mixed": mixed,
+        "adjoint": adjoint,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -90,9 +90,14 @@ typedef struct uw_totals {
     uint32_t overflow; /* 1 if any chunk overflowed */
 } uw_totals;
 
+/* Vertex wire formats. UW_VERTEX_XYZ: 3 floats (world position). UW_VERTEX_XYZ_RGB: 6 floats,
+ * the reference's VertColor (draw.rs:4-9): position, then the sRGB terrain colour of
+ * chunk.rs:215-222 (taken, like the reference, from the edge's first-seen cell). */
+enum uw_vertex_format { UW_VERTEX_XYZ = 0, UW_VERTEX_XYZ_RGB = 1 };
+
 /* Output description for the multi-chunk device entry points. All pointers are device
- * pointers: verts holds 3 floats per vertex (vert_cap vertices), inds ind_cap u32 values,
- * info one record per chunk, totals one record. */
+ * pointers: verts holds 3 (or 6, see vertex_format) floats per vertex (vert_cap vertices), inds
+ * ind_cap u32 values, info one record per chunk, totals one record. */
 typedef struct uw_mesh_out {
     float* verts;
     uint32_t* inds;
@@ -100,6 +105,8 @@ typedef struct uw_mesh_out {
     uw_totals* totals;
     uint64_t vert_cap;
     uint64_t ind_cap;
+    uint32_t vertex_format; /* enum uw_vertex_format */
+    uint32_t reserved;
 } uw_mesh_out;
 
 typedef struct uw_ctx uw_ctx;
@@ -133,9 +140,16 @@ uint64_t uw_max_inds(int32_t n);
 /* Density sampling only (iso_at over every chunk sample, chunk.rs:105-129):
  * coords  device int32[3 * nchunks] chunk coordinates,
  * dens    device float[nchunks * (n+1)^3], per chunk index x (n+1)^2 + y (n+1) + z,
- * below   device uint32[nchunks] or NULL: 1 if some sample is not above iso (not early-blank). */
+ * below   device uint32[nchunks] or NULL: sign summary of each chunk's samples, bit 0 = some
+ *         sample not above iso (so not early-blank), bit 1 = some below iso, bit 2 = some at or
+ *         above iso (a triangle needs bits 1 and 2). */
 int uw_sample_density(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, int32_t n, float* dens,
                       uint32_t* below, void* stream);
+
+/* Density at arbitrary noise-space points: perlin_util::iso_at(perlin, x, y, z)
+ * (perlin_util.rs:24-29), as the boids query it point by point (boid.rs:131-143,324-331).
+ * xyz: device double[3 npts]; out: device float[npts]. */
+int uw_density_at(uw_ctx* ctx, const double* xyz, int64_t npts, float* out, void* stream);
 
 /* Plain path: every stage of the chunk builder as its own kernel launch over all chunks
  * (density -> classify + compact -> offsets -> emit). Uniform resolution n. `cases` may be NULL
@@ -154,13 +168,15 @@ int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t
 #define UW_BATCH_MAX_RESOLUTION 32
 
 /* Launch shape of the persistent kernel (chosen by the autotuner, see autotune.py).
- * block: threads per workgroup (128, 256 or 512); grid: persistent workgroups (0 = as many as
- * fit on the device); slab[n]: cell planes per x-slab for chunks of n cells per axis (n = slab[n]
- * builds the chunk in one slab with one density evaluation; smaller slabs bound LDS use and
- * evaluate the densities twice). */
+ * block: threads per workgroup (128, 192, 256 or 512); grid: persistent workgroups (0 = as many
+ * as fit on the device); max_res: largest resolution a launch accepts (sizes the LDS; larger
+ * chunks are flagged UW_CHUNK_INVALID); slab[n]: cell planes per x-slab for chunks of n cells per
+ * axis (slab[n] = n builds the chunk in one slab with one density evaluation; smaller slabs bound
+ * LDS use and evaluate the densities twice). */
 typedef struct uw_batch_tuning {
     int32_t block;
     int32_t grid;
+    int32_t max_res;
     int8_t slab[UW_BATCH_MAX_RESOLUTION + 1];
 } uw_batch_tuning;
 

@@ -155,7 +155,7 @@ public:
         reserve(n, vcap, icap);
         hip(hipMemcpy(d_coords_, coords.data(), coords.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         hip(hipMemcpy(d_res_, res.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
-        uw_mesh_out out = {d_verts_, d_inds_, d_info_, d_totals_, vcap, icap};
+        uw_mesh_out out = {d_verts_, d_inds_, d_info_, d_totals_, vcap, icap, UW_VERTEX_XYZ, 0u};
         check(uw_build_chunks_batched(perlin_.ctx(), d_coords_, d_res_, int32_t(n), &out, nullptr), perlin_.ctx(),
               "uw_build_chunks_batched");
         hip(hipDeviceSynchronize());

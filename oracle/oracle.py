@@ -79,6 +79,10 @@ def lib():
                                    ctypes.c_int, P(ctypes.c_float), ctypes.c_int64, P(ctypes.c_uint32),
                                    ctypes.c_int64, P(ctypes.c_uint8), P(ctypes.c_uint32), P(UwoMeshCounts)]
         L.uwo_mesh_ref.restype = ctypes.c_int
+        L.uwo_mesh_ref_rgb.argtypes = [P(UwoParams), P(ctypes.c_int32), ctypes.c_int32, P(ctypes.c_float),
+                                       P(ctypes.c_float), P(ctypes.c_float), ctypes.c_int64, P(ctypes.c_uint32),
+                                       ctypes.c_int64, P(UwoMeshCounts)]
+        L.uwo_mesh_ref_rgb.restype = ctypes.c_int
         L.uwo_mesh_fp64.argtypes = [P(UwoParams), P(ctypes.c_int32), ctypes.c_int32, P(ctypes.c_double),
                                     P(ctypes.c_double), ctypes.c_int64, P(ctypes.c_uint32), ctypes.c_int64,
                                     P(ctypes.c_uint8), P(UwoMeshCounts)]
@@ -205,6 +209,36 @@ def mesh_ref(params: dict, coord, n: int, dens: np.ndarray, dedup: str = "hash",
     return out
 
 
+def mesh_ref_rgb(params: dict, coord, n: int, dens: np.ndarray):
+    """mesh_ref with the reference's VertColor colours: (verts (V,3), colors (V,3), inds)."""
+    s = make_params(params)
+    c = np.ascontiguousarray(coord, np.int32)
+    d = np.ascontiguousarray(dens, np.float32)
+    vcap, icap = vert_capacity(n), 15 * n ** 3 + 16
+    verts = np.zeros((vcap, 3), np.float32)
+    colors = np.zeros((vcap, 3), np.float32)
+    inds = np.zeros(icap, np.uint32)
+    cnt = UwoMeshCounts()
+    rc = lib().uwo_mesh_ref_rgb(ctypes.byref(s), _ptr(c, ctypes.c_int32), n, _ptr(d, ctypes.c_float),
+                                _ptr(verts, ctypes.c_float), _ptr(colors, ctypes.c_float), vcap,
+                                _ptr(inds, ctypes.c_uint32), icap, ctypes.byref(cnt))
+    if rc != 0:
+        raise RuntimeError("uwo_mesh_ref_rgb failed: %d" % rc)
+    return verts[:cnt.n_verts].copy(), colors[:cnt.n_verts].copy(), inds[:cnt.n_inds].copy()
+
+
+def vertex_color(world_z: float, corner_b: int, chunk_size: float = 16.0) -> np.ndarray:
+    """The reference's per-vertex terrain colour (chunk.rs:215-222) for one vertex."""
+    L = lib()
+    if not hasattr(L, "_color_ready"):
+        L.uwo_vertex_color.argtypes = [ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_float)]
+        L.uwo_vertex_color.restype = None
+        L._color_ready = True
+    out = np.zeros(3, np.float32)
+    L.uwo_vertex_color(float(world_z), int(corner_b), float(chunk_size), _ptr(out, ctypes.c_float))
+    return out
+
+
 def mesh_fp64(params: dict, coord, n: int, dens64: np.ndarray):
     s = make_params(params)
     c = np.ascontiguousarray(coord, np.int32)
@@ -301,3 +335,16 @@ def plane_loss_dens(n: int, step: float, iso: float, rho64: np.ndarray, cases: n
     loss = L.uwo_plane_loss_dens(n, step, iso, _ptr(r, ctypes.c_double), _ptr(cs, ctypes.c_uint8), target_z,
                                  _ptr(gd, ctypes.c_double) if grads else None)
     return (loss, gd) if grads else loss
+
+
+def iso_at(params: dict, pts: np.ndarray) -> np.ndarray:
+    """perlin_util::iso_at at noise-space points (N, 3): the boids' point query."""
+    L = lib()
+    P = perm(params["seed"])
+    s = make_params(params)
+    pts = np.asarray(pts, np.float64).reshape(-1, 3)
+    f = params["frequency"]
+    off = [float(v) * f for v in params.get("offset", (0.0, 0.0, 0.0))]
+    Pp = _ptr(P, ctypes.c_uint8)
+    return np.array([L.uwo_iso_at(Pp, ctypes.byref(s), x + off[0], y + off[1], z + off[2]) for x, y, z in pts],
+                    np.float32)

@@ -327,6 +327,40 @@ typedef struct {
 } uwo_mesh_counts;
 
 /*
+ * Terrain colour of a vertex (chunk.rs:215-222), f32 in the reference's operation order:
+ *   world_z_ratio = world_z / CHUNK_SIZE; mix = create_mix_ratio(MIN_Z, MAX_Z, ratio)
+ *   (util.rs:93-95, world.rs:11-12: -2, 2); value = BASE_VALUE + (corner_b_idx % 3) / 9;
+ *   hue = MIN_HUE + (MAX_HUE - MIN_HUE) * mix (chunk.rs:14-17: -150, 60, saturation 0.6,
+ *   base value 0.4); hsv_to_rgb (util.rs:122-153: rem_euclid 360, sextant by h = hue / 60,
+ *   times 255) then to_srgb (util.rs:106-111: ((c / 255 + 0.055) / 1.055)^2.4).
+ */
+void uwo_vertex_color(float world_z, int corner_b_idx, float chunk_size, float rgb[3]) {
+    const float ratio = world_z / chunk_size;
+    const float mix = (ratio - (-2.0f)) / (2.0f - (-2.0f));
+    const float value = 0.4f + (float)(corner_b_idx % 3) / 9.0f;
+    float hue = -150.0f + (60.0f - (-150.0f)) * mix;
+    float hr = fmodf(hue, 360.0f);
+    if (hr < 0.0f) hr += 360.0f;
+    const float c = value * 0.6f;
+    const float h = hr / 60.0f;
+    const float x = c * (1.0f - fabsf(fmodf(h, 2.0f) - 1.0f));
+    const float m = value - c;
+    float r, g, b;
+    if (0.0f <= h && h < 1.0f) { r = c; g = x; b = 0.0f; }
+    else if (1.0f <= h && h < 2.0f) { r = x; g = c; b = 0.0f; }
+    else if (2.0f <= h && h < 3.0f) { r = 0.0f; g = c; b = x; }
+    else if (3.0f <= h && h < 4.0f) { r = 0.0f; g = x; b = c; }
+    else if (4.0f <= h && h < 5.0f) { r = x; g = 0.0f; b = c; }
+    else { r = c; g = 0.0f; b = x; }
+    const float lin[3] = {(r + m) * 255.0f, (g + m) * 255.0f, (b + m) * 255.0f};
+    for (int k = 0; k < 3; ++k) rgb[k] = powf((lin[k] / 255.0f + 0.055f) / 1.055f, 2.4f);
+}
+
+static int mesh_ref_impl(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
+                         int dedup_mode, float* verts, float* colors, int64_t vcap, uint32_t* inds,
+                         int64_t icap, uint8_t* cases, uint32_t* tri_cells, uwo_mesh_counts* counts);
+
+/*
  * Reference-faithful mesher. dedup_mode 0 = the reference's literal linear search over the
  * ordered corner pairs seen so far (chunk.rs:233-243, O(V*I)); 1 = the same first-seen
  * numbering through a hash map (identical output, used for large grids).
@@ -339,6 +373,21 @@ typedef struct {
 int uwo_mesh_ref(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
                  int dedup_mode, float* verts, int64_t vcap, uint32_t* inds, int64_t icap,
                  uint8_t* cases, uint32_t* tri_cells, uwo_mesh_counts* counts) {
+    return mesh_ref_impl(prm, coord, n, dens, dedup_mode, verts, NULL, vcap, inds, icap, cases, tri_cells,
+                         counts);
+}
+
+/* uwo_mesh_ref plus colors: 3 floats per vertex, the VertColor colour of the vertex's first-seen
+ * emission (chunk.rs:215-231; the vertex pushed is the first one built for its corner pair). */
+int uwo_mesh_ref_rgb(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
+                     float* verts, float* colors, int64_t vcap, uint32_t* inds, int64_t icap,
+                     uwo_mesh_counts* counts) {
+    return mesh_ref_impl(prm, coord, n, dens, 1, verts, colors, vcap, inds, icap, NULL, NULL, counts);
+}
+
+static int mesh_ref_impl(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
+                         int dedup_mode, float* verts, float* colors, int64_t vcap, uint32_t* inds,
+                         int64_t icap, uint8_t* cases, uint32_t* tri_cells, uwo_mesh_counts* counts) {
     const int32_t L = n + 1;
     const float scale = uwo_step(prm, n);
     const float iso = prm->iso;
@@ -415,6 +464,9 @@ int uwo_mesh_ref(const uwo_params* prm, const int32_t* coord, int32_t n, const f
                             verts[3 * nv + 0] = m[0] + off[0];
                             verts[3 * nv + 1] = m[1] + off[1];
                             verts[3 * nv + 2] = m[2] + off[2];
+                            if (colors)
+                                uwo_vertex_color(verts[3 * nv + 2], ib, (float)prm->chunk_extent,
+                                                 colors + 3 * nv);
                         }
                         ++nv;
                     }

@@ -189,3 +189,18 @@ def test_batched_overflow_and_invalid_resolution():
     empty = build_chunks_batched(perlin, np.zeros((0, 3), np.int32), 12)
     _sync()
     assert empty.totals()["n_inds"] == 0
+
+
+def test_max_resolution_guard():
+    from underwaterworld_amd import Perlin, _lib, build_chunks_batched, get_batch_tuning, set_batch_tuning
+    perlin = Perlin(PR.preset("chunks12"))
+    t = get_batch_tuning(perlin)
+    t["max_res"] = 12
+    set_batch_tuning(perlin, t)
+    coords = [(0, 0, -1), (3, 1, -1), (0, 0, -1)]
+    b = build_chunks_batched(perlin, coords, [12, 24, 8])
+    _sync()
+    info = b.info()
+    assert info[1]["flags"] & _lib.CHUNK_INVALID
+    assert not info[0]["flags"] & _lib.CHUNK_INVALID and info[0]["n_inds"] > 0
+    assert not info[2]["flags"] & _lib.CHUNK_INVALID

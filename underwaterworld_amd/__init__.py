@@ -6,7 +6,8 @@ interface (noise::Perlin + Chunk) and the device-resident multi-chunk forms.
 """
 from . import params
 from .engine import (BatchedGraph, Chunk, MeshBatch, Perlin, batch_capacity, build_chunks_batched,
-                     build_chunks_plain, get_batch_tuning, permutation_table, sample_density, set_batch_tuning)
+                     build_chunks_plain, density_at, get_batch_tuning, permutation_table, sample_density,
+                     set_batch_tuning)
 
 __all__ = ["params", "Perlin", "Chunk", "MeshBatch", "BatchedGraph", "build_chunks_plain", "build_chunks_batched",
-           "batch_capacity", "sample_density", "permutation_table", "get_batch_tuning", "set_batch_tuning"]
+           "batch_capacity", "sample_density", "density_at", "permutation_table", "get_batch_tuning", "set_batch_tuning"]

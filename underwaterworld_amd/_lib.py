@@ -61,13 +61,14 @@ CHUNK_INVALID = 8
 
 
 class UwBatchTuning(ctypes.Structure):
-    _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32),
+    _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32), ("max_res", ctypes.c_int32),
                 ("slab", ctypes.c_int8 * (BATCH_MAX_RESOLUTION + 1))]
 
 
 class UwMeshOut(ctypes.Structure):
     _fields_ = [("verts", c_void_p), ("inds", c_void_p), ("info", c_void_p), ("totals", c_void_p),
-                ("vert_cap", ctypes.c_uint64), ("ind_cap", ctypes.c_uint64)]
+                ("vert_cap", ctypes.c_uint64), ("ind_cap", ctypes.c_uint64), ("vertex_format", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
 
 
 # name -> (restype, argtypes); every symbol include/uwmesh.h declares.
@@ -95,6 +96,7 @@ SIGNATURES = {
                                                P(c_void_p)]),
     "uw_graph_launch": (ctypes.c_int, [c_void_p, c_void_p]),
     "uw_ctx_set_field": (ctypes.c_int, [c_void_p, c_void_p, c_void_p]),
+    "uw_density_at": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p]),
     "uw_build_chunks_batched_tape": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut),
                                                     c_void_p, c_void_p]),
     "uw_adjoint_backward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p,

@@ -4,12 +4,12 @@
 
 For each workload (the 4096-chunk 12^3 grid with 4 octaves, the 8192-chunk mixed-resolution set
 with 6 octaves) it times, on the device, every combination of
-    block   threads per workgroup: 128, 256, 512
+    block   threads per workgroup: 128, 192, 256, 512
     budget  LDS bytes allowed for a chunk's density slab ring: a chunk whose (n+1)^3 densities fit
             is built in one slab (one density evaluation); larger chunks use the largest x-slab
             that fits (densities evaluated twice)
-    grid    persistent workgroups: all that fit (0), or a fixed multiple of the CU count
-and writes the fastest shape per octave count to tuning.json, which `Perlin` applies when it
+    grid    persistent workgroups: all that fit (0)
+with max_res set to the workload's largest resolution (it sizes the launch's LDS), and writes the fastest shape per octave count to tuning.json, which `Perlin` applies when it
 creates a context. Every candidate's mesh is checked against the first candidate's (all shapes
 must produce identical meshes) before its time is accepted.
 """
@@ -25,7 +25,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning.json")
 
-BLOCKS = (128, 256, 512)
+BLOCKS = (128, 192, 256, 512)
 BUDGETS = (6 * 1024, 10 * 1024, 14 * 1024, 20 * 1024, 28 * 1024, 40 * 1024, 56 * 1024)
 GRIDS = (0,)
 
@@ -64,7 +64,7 @@ def tune_workload(perlin, coords, res, steps=10, quick=False, log=print):
     for block in blocks:
         for budget in budgets:
             for grid in GRIDS:
-                t = dict(base, block=block, grid=grid, slab=slabs_for_budget(budget))
+                t = dict(base, block=block, grid=grid, max_res=int(max(res)), slab=slabs_for_budget(budget))
                 try:
                     set_batch_tuning(perlin, t)
                     g = BatchedGraph(perlin, coords, res)

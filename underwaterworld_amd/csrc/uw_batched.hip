@@ -28,7 +28,7 @@ constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
 constexpr int kMaxL = kMaxN + 1;
 constexpr int kLPad = (kMaxL + 3) & ~3;  // ADJ/POST rows padded to keep later tables 16-byte aligned
 constexpr int kRing = 2048;       // cell ring (vertex base << 8 | case), power of two
-constexpr int kListCap = 512;     // compacted active cells pending emission
+constexpr int kMinList = 512;     // smallest compacted active-cell list
 constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
 
 struct BatchArgs {
@@ -44,7 +44,10 @@ struct BatchArgs {
     float* verts;
     uint32_t* inds;
     uw_vertex_tape* tape;  // optional per-vertex record for the adjoint
+    uint32_t vstride;      // floats per vertex: 3 (xyz) or 6 (xyz + sRGB colour)
     uint64_t vcap, icap;
+    int32_t max_res;      // largest resolution accepted by this launch (sizes the LDS)
+    int32_t list_cap;     // entries of the compacted active-cell list
     int8_t sx[kMaxN + 1]; // cell planes per slab for each resolution
 };
 
@@ -144,14 +147,72 @@ __device__ __forceinline__ CellCounts classify_cell(const float* ring, int32_t n
 }
 
 // LDS carve-up (bytes): fixed part, then the per-octave axis tables, then the density ring.
-__host__ __device__ constexpr size_t lds_fixed_bytes() {
-    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + size_t(kRing) * 4 + 2 * size_t(kListCap) * 4 +
+__host__ __device__ constexpr size_t lds_fixed_bytes(size_t list_cap) {
+    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + size_t(kRing) * 4 + 2 * list_cap * 4 +
            2 * kLPad * 4 /*ADJ, POST*/ + 256 /*P*/;
 }
 __host__ __device__ inline size_t lds_table_bytes(int octaves) { return 3 * size_t(octaves) * kMaxL * sizeof(uw_axis); }
 
+// Emission of one active cell: owned-edge lerps into verts (and the adjoint tape), then the
+// cell's indices. vbase: chunk-local vertex id of the cell's first owned record; ibase: global
+// index of its first index; owner_voff(q) gives the chunk-local vertex base of cell q; RING holds
+// (local base << 8 | case) per cell.
+template <class OwnerVoff>
+__device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t L, int32_t x, int32_t y, int32_t z,
+                                          uint32_t cs, const float r[8], uint32_t myv, uint64_t vo, uint64_t ib,
+                                          float step, float ox, float oy, float oz, const uint64_t* TRI,
+                                          const uint32_t* RING, uint32_t ring_mask, OwnerVoff owner_voff) {
+    const float iso = a.f.iso;
+    const uw_tables* tab = a.tab;
+    const uint32_t owned = uw_owned_mask(x, y, z);
+    const uint64_t row = TRI[cs];
+    const int nidx = 3 * int(row & 15u);
+    const uint64_t vb = vo + myv;
+    uint32_t seen = 0;
+    for (int j = 0; j < nidx; ++j) {
+        const uint32_t ed = uint32_t(row >> (4 + 4 * j)) & 15u;
+        const uint32_t bit = 1u << ed;
+        uint32_t id;
+        if (owned & bit) {
+            const uint32_t rank = __popc(uint32_t(tab->before[cs][ed]) & owned);
+            id = myv + rank;
+            if (!(seen & bit)) {
+                float p[3];
+                uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
+                float* dst = a.verts + a.vstride * (vb + rank);
+                dst[0] = p[0];
+                dst[1] = p[1];
+                dst[2] = p[2];
+                if (a.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
+                if (a.tape) {
+                    const uint32_t ia = uw_edge_a(ed), ib2 = uw_edge_b(ed);
+                    const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib2);
+                    uw_vertex_tape tv;
+                    tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L + z +
+                                    int32_t((ca >> 2) & 1u));
+                    tv.b = uint32_t(((x + int32_t(cb & 1u)) * L + y + int32_t((cb >> 1) & 1u)) * L + z +
+                                    int32_t((cb >> 2) & 1u));
+                    tv.rho_a = r[ia];
+                    tv.rho_b = r[ib2];
+                    a.tape[vb + rank] = tv;
+                }
+            }
+        } else {
+            int32_t qx, qy, qz;
+            uint32_t qe;
+            uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
+            const uint32_t q = uint32_t((qx * n + qy) * n + qz);
+            const uint32_t qw = RING[q & ring_mask];
+            id = owner_voff(q, qw) + __popc(uint32_t(tab->before[qw & 255u][qe]) & uw_owned_mask(qx, qy, qz));
+        }
+        seen |= bit;
+        a.inds[ib + j] = id;
+    }
+}
+
 template <int BLOCK, int OMAX>
 __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
+    constexpr int W = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     const int32_t O = a.f.octaves;
     float4* GP = reinterpret_cast<float4*>(smem);
@@ -159,20 +220,21 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     uint32_t* LUT = reinterpret_cast<uint32_t*>(TRI + 256);
     uint32_t* RING = LUT + 256;
     uint32_t* LIST = RING + kRing;
-    uint32_t* LIOFF = LIST + kListCap;
-    float* ADJ = reinterpret_cast<float*>(LIOFF + kListCap);
+    uint32_t* LIOFF = LIST + a.list_cap;
+    float* ADJ = reinterpret_cast<float*>(LIOFF + a.list_cap);
     float* POST = ADJ + kLPad;
     uint8_t* P = reinterpret_cast<uint8_t*>(POST + kLPad);
     uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);
     uw_axis* AY = AX + O * kMaxL;
     uw_axis* AZ = AY + O * kMaxL;
     float* DENS = reinterpret_cast<float*>(AZ + O * kMaxL);
-    __shared__ unsigned long long scan_lds[BLOCK / 64];
-    __shared__ uint32_t s_task;
+    __shared__ unsigned long long scan_lds[W];
+    __shared__ uint32_t wtot[W][3];
+    __shared__ uint32_t s_task, s_bits;
     __shared__ unsigned long long s_vo, s_io;
     __shared__ int s_emit;
 
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uw_tables* tab = a.tab;
     for (int i = tid; i < 256; i += BLOCK) {
         const uint8_t p = tab->perm[i];
@@ -184,25 +246,28 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     const float iso = a.f.iso;
 
     for (;;) {
-        if (tid == 0) s_task = atomicAdd(a.head, 1u);
+        __syncthreads();  // every wave is done with the previous task's shared state
+        if (tid == 0) {
+            s_task = atomicAdd(a.head, 1u);
+            s_bits = 0u;
+        }
         __syncthreads();
         const uint32_t task = s_task;
         if (task >= uint32_t(a.nchunks)) break;
         const int32_t c = int32_t(a.order[task]);
         const int32_t n = a.res[c];
-        if (n < 1 || n > kMaxN || O > OMAX) {
+        if (n < 1 || n > a.max_res || O > OMAX) {
             if (tid == 0) {
                 uw_chunk_info r = {UW_CHUNK_INVALID | UW_CHUNK_BLANK, 0, 0, 0, 0, 0};
                 a.info[c] = r;
             }
-            __syncthreads();
             continue;
         }
-        const int32_t L = n + 1, nn = n * n;
+        const int32_t L = n + 1, nn = n * n, C = nn * n;
         const uw_divider dn = uw_make_divider(uint32_t(n)), dnn = uw_make_divider(uint32_t(nn));
         const int32_t cx = a.coords[3 * c], cy = a.coords[3 * c + 1], cz = a.coords[3 * c + 2];
         const float step = float(a.f.extent) / float(n);
-        // chunk tables use row stride L (not kMaxL): [O][L] per axis
+        const float ox = float(a.f.extent * cx), oy = float(a.f.extent * cy), oz = float(a.f.extent * cz);
         const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
         uw_build_chunk_tables(a.f, P, cx, cy, cz, L, step, T, tid, BLOCK);
         __syncthreads();
@@ -210,14 +275,137 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         const int32_t sx = a.sx[n];
         const int32_t ns = sx + 1;
         const int32_t nslab = (n + sx - 1) / sx;
-        const bool single = nslab == 1;
 
-        // ---- counting pass ----
+        if (nslab == 1 && C <= a.list_cap) {
+            // ---------- one density evaluation, one classification pass ----------
+            const int bits = uw_noise_planes<OMAX>(a.f, T, P, GP, L, 0, L, DENS, ns, tid, BLOCK);
+            if (bits) atomicOr(&s_bits, uint32_t(bits));
+            __syncthreads();
+            const uint32_t sb = s_bits;
+            const bool straddle = (sb & UW_BELOW_LT) && (sb & UW_BELOW_GE);
+            if (!straddle) {
+                if (tid == 0) {
+                    const uint32_t fl = (sb & UW_BELOW_NOT_ABOVE) ? uint32_t(UW_CHUNK_BLANK)
+                                                                : uint32_t(UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP);
+                    uw_chunk_info r = {fl, 0, 0, 0, 0, 0};
+                    a.info[c] = r;
+                }
+                continue;
+            }
+            // wave w owns cells [w cpw, (w+1) cpw): wave-level scans, no block barrier
+            const uint32_t cpw = (uint32_t(C) + W - 1) / W;
+            const uint32_t c0 = uint32_t(wave) * cpw, c1 = min(uint32_t(C), c0 + cpw);
+            uint32_t cv = 0, ct = 0, ca = 0;
+            for (uint32_t base = c0; base < c1; base += 64) {
+                const uint32_t g = base + lane;
+                CellCounts cc = {0, 0, 0, 0};
+                if (g < c1) {
+                    int32_t x, y, z;
+                    uw_cell_xyz(g, dn, dnn, &x, &y, &z);
+                    cc = classify_cell(DENS, ns, L, x, y, z, iso, LUT);
+                }
+                const unsigned long long v = (unsigned long long)cc.nv | ((unsigned long long)cc.nt << 16) |
+                                             ((unsigned long long)cc.act << 32);
+                unsigned long long inc = v;
+#pragma unroll
+                for (int d = 1; d < 64; d <<= 1) {
+                    const unsigned long long t = __shfl_up(inc, d, 64);
+                    if (lane >= d) inc += t;
+                }
+                const unsigned long long ex = inc - v;
+                if (g < c1) {
+                    RING[g] = ((cv + uint32_t(ex & 0xFFFFull)) << 8) | cc.cs;
+                    if (cc.act) {
+                        const uint32_t slot = c0 + ca + uint32_t(ex >> 32);
+                        LIST[slot] = g;
+                        LIOFF[slot] = 3u * (ct + uint32_t((ex >> 16) & 0xFFFFull));
+                    }
+                }
+                const unsigned long long tot = __shfl(inc, 63, 64);
+                cv += uint32_t(tot & 0xFFFFull);
+                ct += uint32_t((tot >> 16) & 0xFFFFull);
+                ca += uint32_t(tot >> 32);
+            }
+            if (lane == 0) {
+                wtot[wave][0] = cv;
+                wtot[wave][1] = ct;
+                wtot[wave][2] = ca;
+            }
+            __syncthreads();
+            uint32_t vbase[W], ibase[W], abase[W + 1];
+            uint32_t nv = 0, nt = 0, na = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                vbase[w] = nv;
+                ibase[w] = 3u * nt;
+                abase[w] = na;
+                nv += wtot[w][0];
+                nt += wtot[w][1];
+                na += wtot[w][2];
+            }
+            abase[W] = na;
+            const uint32_t ni = 3u * nt;
+            if (tid == 0) {
+                uint32_t flags = ni == 0 ? uint32_t(UW_CHUNK_BLANK) : 0u;
+                unsigned long long vo = 0, io = 0;
+                if (ni > 0) {
+                    vo = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_verts), (unsigned long long)nv);
+                    io = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_inds), (unsigned long long)ni);
+                    atomicAdd(&a.totals->n_nonblank, 1u);
+                    if (vo + nv > a.vcap || io + ni > a.icap) {
+                        flags |= UW_CHUNK_OVERFLOW;
+                        atomicOr(&a.totals->overflow, 1u);
+                    }
+                }
+                uw_chunk_info r = {flags, nv, ni, na, vo, io};
+                a.info[c] = r;
+                s_vo = vo;
+                s_io = io;
+                s_emit = (ni > 0 && !(flags & UW_CHUNK_OVERFLOW)) ? 1 : 0;
+            }
+            __syncthreads();
+            if (!s_emit) continue;
+            const unsigned long long vo = s_vo, io = s_io;
+            const uw_divider dcpw = uw_make_divider(cpw);
+            auto owner_voff = [&](uint32_t q, uint32_t qw) -> uint32_t {
+                const uint32_t wq = uw_div(q, dcpw);
+                uint32_t b = 0;
+#pragma unroll
+                for (int w = 0; w < W; ++w) b = (uint32_t(w) == wq) ? vbase[w] : b;
+                return b + (qw >> 8);
+            };
+            for (uint32_t e = tid; e < na; e += BLOCK) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int k = 1; k < W; ++k) w += (e >= abase[k]) ? 1u : 0u;
+                uint32_t wb = 0, wi = 0, wa = 0;
+#pragma unroll
+                for (int k = 0; k < W; ++k)
+                    if (uint32_t(k) == w) {
+                        wb = vbase[k];
+                        wi = ibase[k];
+                        wa = abase[k];
+                    }
+                const uint32_t slot = w * cpw + (e - wa);
+                const uint32_t g = LIST[slot];
+                const uint32_t rw = RING[g];
+                int32_t x, y, z;
+                uw_cell_xyz(g, dn, dnn, &x, &y, &z);
+                float r[8];
+                cell_corners(DENS, ns, L, x, y, z, r);
+                emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + LIOFF[slot], step, ox, oy,
+                          oz, TRI, RING, 0xFFFFFFFFu, owner_voff);
+            }
+            continue;
+        }
+
+        // ---------- x-slabs: counting pass, reservation, emission pass ----------
         int below = 0;
         unsigned long long cnt = 0;  // nv | nt << 24 | act << 48 (per thread)
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
-            below |= uw_noise_planes<OMAX>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+            below |= uw_noise_planes<OMAX>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
+                     UW_BELOW_NOT_ABOVE;
             __syncthreads();
             for (int32_t g = xa * nn + tid; g < xe * nn; g += BLOCK) {
                 int32_t x, y, z;
@@ -226,7 +414,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 cnt += (unsigned long long)cc.nv | ((unsigned long long)cc.nt << 24) |
                        ((unsigned long long)cc.act << 48);
             }
-            if (!single) __syncthreads();
+            if (nslab > 1) __syncthreads();
         }
         below = __syncthreads_or(below);
         unsigned long long tot;
@@ -257,15 +445,15 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         __syncthreads();
         if (!s_emit) continue;
 
-        // ---- emission pass ----
         const unsigned long long vo = s_vo, io = s_io;
-        const float ox = float(a.f.extent * cx), oy = float(a.f.extent * cy), oz = float(a.f.extent * cz);
         // rounds that may stay pending before the cell ring would overwrite an owner
         const int32_t max_pending_rounds = max(1, (kRing - nn - 1) / BLOCK - 1);
+        const uint32_t list_limit = uint32_t(a.list_cap - BLOCK);
+        auto owner_voff = [](uint32_t, uint32_t qw) -> uint32_t { return qw >> 8; };
         uint32_t carry_v = 0, carry_i = 0;
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
-            if (!single) {
+            if (nslab > 1) {
                 uw_noise_planes<OMAX>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
                 __syncthreads();
             }
@@ -297,61 +485,17 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 list_n += uint32_t(rt >> 32);
                 ++pending_rounds;
                 const bool last = g0 + BLOCK >= gend;
-                if (last || list_n > uint32_t(kListCap - BLOCK) || pending_rounds >= max_pending_rounds) {
+                if (last || list_n > list_limit || pending_rounds >= max_pending_rounds) {
                     __syncthreads();
                     for (uint32_t e = tid; e < list_n; e += BLOCK) {
-                        const int32_t gg = int32_t(LIST[e]);
+                        const uint32_t gg = LIST[e];
                         const uint32_t w = RING[gg & (kRing - 1)];
-                        const uint32_t cs = w & 255u, myv = w >> 8;
                         int32_t x, y, z;
-                        uw_cell_xyz(uint32_t(gg), dn, dnn, &x, &y, &z);
+                        uw_cell_xyz(gg, dn, dnn, &x, &y, &z);
                         float r[8];
                         cell_corners(DENS, ns, L, x, y, z, r);
-                        const uint32_t owned = uw_owned_mask(x, y, z);
-                        const uint64_t row = TRI[cs];
-                        const int nidx = 3 * int(row & 15u);
-                        const uint64_t vb = vo + myv;
-                        const uint64_t ib = io + LIOFF[e];
-                        uint32_t seen = 0;
-                        for (int j = 0; j < nidx; ++j) {
-                            const uint32_t ed = uint32_t(row >> (4 + 4 * j)) & 15u;
-                            const uint32_t bit = 1u << ed;
-                            uint32_t id;
-                            if (owned & bit) {
-                                const uint32_t rank = __popc(uint32_t(tab->before[cs][ed]) & owned);
-                                id = myv + rank;
-                                if (!(seen & bit)) {
-                                    float p[3];
-                                    uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
-                                    float* dst = a.verts + 3 * (vb + rank);
-                                    dst[0] = p[0];
-                                    dst[1] = p[1];
-                                    dst[2] = p[2];
-                                    if (a.tape) {
-                                        const uint32_t ia = uw_edge_a(ed), ib = uw_edge_b(ed);
-                                        const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib);
-                                        uw_vertex_tape tv;
-                                        tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L +
-                                                        z + int32_t((ca >> 2) & 1u));
-                                        tv.b = uint32_t(((x + int32_t(cb & 1u)) * L + y + int32_t((cb >> 1) & 1u)) * L +
-                                                        z + int32_t((cb >> 2) & 1u));
-                                        tv.rho_a = r[ia];
-                                        tv.rho_b = r[ib];
-                                        a.tape[vb + rank] = tv;
-                                    }
-                                }
-                            } else {
-                                int32_t qx, qy, qz;
-                                uint32_t qe;
-                                uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
-                                const int32_t q = (qx * n + qy) * n + qz;
-                                const uint32_t qw = RING[q & (kRing - 1)];
-                                id = (qw >> 8) +
-                                     __popc(uint32_t(tab->before[qw & 255u][qe]) & uw_owned_mask(qx, qy, qz));
-                            }
-                            seen |= bit;
-                            a.inds[ib + j] = id;
-                        }
+                        emit_cell(a, n, L, x, y, z, w & 255u, r, w >> 8, vo, io + LIOFF[e], step, ox, oy, oz, TRI,
+                                  RING, uint32_t(kRing - 1), owner_voff);
                     }
                     __syncthreads();
                     list_n = 0;
@@ -374,6 +518,7 @@ BatchedKernel pick_octaves(int omax) {
 BatchedKernel pick_kernel(int block, int omax) {
     switch (block) {
         case 128: return pick_octaves<128>(omax);
+        case 192: return pick_octaves<192>(omax);
         case 512: return pick_octaves<512>(omax);
         default: return pick_octaves<256>(omax);
     }
@@ -381,9 +526,18 @@ BatchedKernel pick_kernel(int block, int omax) {
 
 int omax_of(int octaves) { return octaves <= 4 ? 4 : (octaves <= 8 ? 8 : 16); }
 
+// Compacted-list capacity: every chunk built in one pass (one slab, n^3 <= kRing cells) must fit,
+// and a round of `block` cells must fit on top of a flushable list.
+int32_t list_cap_for(const uw_batch_tuning& t) {
+    int32_t cap = kMinList > t.block ? kMinList : t.block;
+    for (int n = 1; n <= t.max_res; ++n)
+        if (t.slab[n] == n && n * n * n <= kRing) cap = n * n * n > cap ? n * n * n : cap;
+    return cap > kRing ? kRing : cap;
+}
+
 size_t dens_bytes_for(const uw_batch_tuning& t) {
     size_t m = 0;
-    for (int n = 1; n <= kMaxN; ++n) {
+    for (int n = 1; n <= t.max_res; ++n) {
         const size_t L = size_t(n) + 1;
         const size_t ns = size_t(t.slab[n]) + 1;
         m = ns * L * L * 4 > m ? ns * L * L * 4 : m;
@@ -399,6 +553,7 @@ void uw_default_batch_tuning(uw_batch_tuning* t) {
     memset(t, 0, sizeof(*t));
     t->block = 256;
     t->grid = 0;
+    t->max_res = kMaxN;
     for (int n = 1; n <= kMaxN; ++n) {
         const int L = n + 1;
         int sx = n;
@@ -408,8 +563,8 @@ void uw_default_batch_tuning(uw_batch_tuning* t) {
 }
 
 int uw_check_batch_tuning(const uw_batch_tuning& t) {
-    if (t.block != 128 && t.block != 256 && t.block != 512) return UW_ERR_INVALID_ARG;
-    if (t.grid < 0) return UW_ERR_INVALID_ARG;
+    if (t.block != 128 && t.block != 192 && t.block != 256 && t.block != 512) return UW_ERR_INVALID_ARG;
+    if (t.grid < 0 || t.max_res < 1 || t.max_res > kMaxN) return UW_ERR_INVALID_ARG;
     for (int n = 1; n <= kMaxN; ++n)
         if (t.slab[n] < 1 || t.slab[n] > n) return UW_ERR_INVALID_ARG;
     return UW_OK;
@@ -437,7 +592,7 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
     }
     if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, sizeof(uint32_t)));
     bl->k = pick_kernel(tune.block, omax_of(octaves));
-    bl->lds = lds_fixed_bytes() + lds_table_bytes(octaves) + dens_bytes_for(tune);
+    bl->lds = lds_fixed_bytes(size_t(list_cap_for(tune))) + lds_table_bytes(octaves) + dens_bytes_for(tune);
     UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(bl->k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(bl->lds)));
     bl->grid = tune.grid;
@@ -469,6 +624,9 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.vcap = out->vert_cap;
     a.icap = out->ind_cap;
     a.tape = tape;
+    a.vstride = out->vertex_format == UW_VERTEX_XYZ_RGB ? 6u : 3u;
+    a.max_res = tune.max_res;
+    a.list_cap = list_cap_for(tune);
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
     a.order = order;
     a.head = head;
@@ -482,7 +640,7 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
 }
 
 bool args_ok(int32_t nchunks, const int32_t* coords, const int32_t* res, const uw_mesh_out* out) {
-    return nchunks >= 0 && out && out->info && out->totals && (nchunks == 0 || (coords && res)) &&
+    return nchunks >= 0 && out && out->info && out->totals && out->vertex_format <= 1 && (nchunks == 0 || (coords && res)) &&
            (out->vert_cap == 0 || out->verts) && (out->ind_cap == 0 || out->inds);
 }
 

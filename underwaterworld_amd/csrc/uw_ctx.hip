@@ -142,6 +142,11 @@ uw_field uw_make_field(const uw_params& p) {
         sum += p.amplitudes[k];
     }
     f.amp_sum = float(sum);
+    // 1/(sqrt(3)/2) and the normalisation folded into one weight per octave (f64, rounded once)
+    const double scale = 1.1547005383792515;
+    for (int k = 0; k < p.octaves; ++k) f.weight[k] = float(p.amplitudes[k] * scale / sum);
+    f.clamp_hi = float(1.0 / scale);
+    f.clamp_lo = -f.clamp_hi;
     f.octaves = p.octaves;
     f.extent = p.chunk_extent;
     f.bias_kind = p.bias_kind;

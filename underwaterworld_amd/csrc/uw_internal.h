@@ -93,6 +93,8 @@ struct uw_field {
     double octave_freq[UW_MAX_OCTAVES];  // lacunarity^k by repeated multiplication
     float amp[UW_MAX_OCTAVES];           // amplitudes rounded to f32
     float amp_sum;                       // sum of amplitudes (f64 sum rounded to f32)
+    float weight[UW_MAX_OCTAVES];        // A_k * (2/sqrt 3) / sum A, rounded once to f32
+    float clamp_lo, clamp_hi;            // -+ sqrt(3)/2: the crate's [-1, 1] clamp before scaling
     int32_t octaves;
     int32_t extent;
     int32_t bias_kind;

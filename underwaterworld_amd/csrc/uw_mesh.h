@@ -95,6 +95,33 @@ __device__ __forceinline__ uint32_t uw_case_of(const float r[8], float iso) {
     return c;
 }
 
+// Vertex colour of the reference's terrain (chunk.rs:215-222), f32 in the reference's order:
+// hue from the world height (create_mix_ratio over world.rs MIN_Z..MAX_Z = -2..2 in chunk units,
+// util.rs:93-95), value from the edge's end corner index mod 3, hsv_to_rgb (util.rs:122-153,
+// saturation 0.6, base value 0.4, chunk.rs:14-17), then to_srgb (util.rs:106-111).
+__device__ __forceinline__ void uw_vertex_color(float world_z, uint32_t corner_b, float extent, float rgb[3]) {
+    const float ratio = world_z / extent;
+    const float mix = (ratio - (-2.0f)) / (2.0f - (-2.0f));
+    const float value = 0.4f + float(corner_b % 3u) / 9.0f;
+    float hue = -150.0f + (60.0f - (-150.0f)) * mix;
+    hue = fmodf(hue, 360.0f);
+    if (hue < 0.0f) hue = hue + 360.0f;
+    const float c = value * 0.6f;
+    const float h = hue / 60.0f;
+    const float x = c * (1.0f - fabsf(fmodf(h, 2.0f) - 1.0f));
+    const float m = value - c;
+    float r, g, b;
+    if (0.0f <= h && h < 1.0f) { r = c; g = x; b = 0.0f; }
+    else if (1.0f <= h && h < 2.0f) { r = x; g = c; b = 0.0f; }
+    else if (2.0f <= h && h < 3.0f) { r = 0.0f; g = c; b = x; }
+    else if (3.0f <= h && h < 4.0f) { r = 0.0f; g = x; b = c; }
+    else if (4.0f <= h && h < 5.0f) { r = x; g = 0.0f; b = c; }
+    else { r = c; g = 0.0f; b = x; }
+    const float lin[3] = {(r + m) * 255.0f, (g + m) * 255.0f, (b + m) * 255.0f};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rgb[i] = powf((lin[i] / 255.0f + 0.055f) / 1.055f, 2.4f);
+}
+
 // World position of the lerp on edge e of cell (x, y, z) (chunk.rs:185-229), f32 with the
 // reference's operation order (the file is compiled with -ffp-contract=off).
 __device__ __forceinline__ void uw_edge_vertex(uint32_t e, int32_t x, int32_t y, int32_t z, const float r[8],

@@ -144,11 +144,12 @@ __device__ __forceinline__ uw_line_oct uw_line_setup(const uw_axis& X, const uw_
     return s;
 }
 
-__device__ __forceinline__ float uw_line_sample(const uw_line_oct& s, const uw_axis& Z) {
+// Unscaled octave value at the line sample, clamped to the unscaled image of [-1, 1]; the caller
+// multiplies by the octave weight A_k (2/sqrt 3) / sum A.
+__device__ __forceinline__ float uw_line_sample(const uw_line_oct& s, const uw_axis& Z, float lo, float hi) {
     const float n0 = fmaf(s.B0, Z.d, s.A0);
     const float n1 = fmaf(s.B1, Z.dm1, s.A1);
-    const float r = uw_lerp(Z.fade, n0, n1) * UW_PERLIN_SCALE;
-    return fminf(fmaxf(r, -1.0f), 1.0f);
+    return __builtin_amdgcn_fmed3f(uw_lerp(Z.fade, n0, n1), lo, hi);
 }
 
 // Per-chunk LDS tables: every (axis index, octave) entry of the chunk and the z-bias split.
@@ -177,9 +178,15 @@ __device__ __forceinline__ void uw_build_chunk_tables(const uw_field& f, const u
     for (int i = tid; i < L; i += nthr) uw_bias_terms(f, uw_sample_coord(f, 2, cz, i, step), &T.ADJ[i], &T.POST[i]);
 }
 
+// Sign summary bits of a set of samples: some sample not above iso (so not early-blank), some
+// below iso, some at or above iso (a cell can straddle iso only if both of the last two hold).
+#define UW_BELOW_NOT_ABOVE 1
+#define UW_BELOW_LT 2
+#define UW_BELOW_GE 4
+
 // Densities of x-planes [xa, xb) of a chunk whose tables are in T, one thread per (x, y) line.
 // Plane x is written to dst + slot(x) * L * L with slot(x) = x % nslot, z fastest.
-// Returns, per thread, whether some sample was not above iso.
+// Returns, per thread, the OR of the UW_BELOW_* bits of its samples.
 template <int OMAX>
 __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uint8_t* P,
                                                const float4* GP, int32_t L, int32_t xa, int32_t xb, float* dst,
@@ -213,12 +220,13 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
                         st[k] = uw_line_setup(T.AX[k * L + ix], T.AY[k * L + iy], bb[k], Z.bits, GP);
                         zcur[k] = Z.bits;
                     }
-                    total = fmaf(f.amp[k], uw_line_sample(st[k], Z), total);
+                    total = fmaf(f.weight[k], uw_line_sample(st[k], Z, f.clamp_lo, f.clamp_hi), total);
                 }
             }
-            const float rho = uw_density_from(total, f.amp_sum, T.ADJ[iz], T.POST[iz]);
+            const float rho = (T.ADJ[iz] + total) + T.POST[iz];
             out[iz] = rho;
-            below |= !(rho > f.iso);
+            below |= (rho > f.iso ? 0 : UW_BELOW_NOT_ABOVE) | (rho < f.iso ? UW_BELOW_LT : 0) |
+                     (rho >= f.iso ? UW_BELOW_GE : 0);
         }
     }
     return below;

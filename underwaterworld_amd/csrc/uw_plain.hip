@@ -49,7 +49,9 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     uw_axis* AZ = AY + O * L;
     float* SLAB = reinterpret_cast<float*>(AZ + O * L);            // [planes][L][L]
 
+    __shared__ uint32_t s_bits;
     const int32_t cx = coords[3 * chunk], cy = coords[3 * chunk + 1], cz = coords[3 * chunk + 2];
+    if (threadIdx.x == 0) s_bits = 0u;
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
@@ -60,14 +62,14 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     uw_build_chunk_tables(f, P, cx, cy, cz, L, step, T, threadIdx.x, blockDim.x);
     __syncthreads();
     // plane x lands in slot x % planes = x - x0
-    int any_below = uw_noise_planes<OMAX>(f, T, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
+    const int bits = uw_noise_planes<OMAX>(f, T, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
+    if (bits) atomicOr(&s_bits, uint32_t(bits));
     __syncthreads();
     const int32_t LL = L * L;
     const int32_t count = nx * LL;
     float* dst = dens + uint64_t(chunk) * uint64_t(LL) * uint64_t(L) + uint64_t(x0) * uint64_t(LL);
     for (int32_t s = threadIdx.x; s < count; s += blockDim.x) dst[s] = SLAB[s];
-    any_below = __syncthreads_or(any_below);
-    if (threadIdx.x == 0 && any_below) atomicOr(&below[chunk], 1u);
+    if (threadIdx.x == 0 && s_bits) atomicOr(&below[chunk], s_bits);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -102,7 +104,7 @@ __global__ void __launch_bounds__(kThreads) k_classify(float iso, const uw_table
     __shared__ uint32_t lut[256];  // used-edge mask | triangle count << 16
     __shared__ uint32_t waves[kThreads / 64];
     const int32_t chunk = blockIdx.x;
-    if (!below[chunk]) {
+    if (!(below[chunk] & UW_BELOW_NOT_ABOVE)) {
         if (threadIdx.x == 0) {
             uw_chunk_info r = {UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP, 0, 0, 0, 0, 0};
             info[chunk] = r;
@@ -220,7 +222,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                                                    const uint32_t* __restrict__ voff, const uint32_t* __restrict__ act,
                                                    const uint32_t* __restrict__ ioff,
                                                    const uw_chunk_info* __restrict__ info, float* __restrict__ verts,
-                                                   uint32_t* __restrict__ inds) {
+                                                   uint32_t* __restrict__ inds, uint32_t vstride) {
     const int32_t chunk = blockIdx.x;
     const uw_chunk_info ci = info[chunk];
     if (ci.flags & (UW_CHUNK_BLANK | UW_CHUNK_OVERFLOW)) return;
@@ -257,10 +259,11 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                 if (!(seen & bit)) {
                     float p[3];
                     uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
-                    float* dst = verts + 3 * (vb + rank);
+                    float* dst = verts + vstride * (vb + rank);
                     dst[0] = p[0];
                     dst[1] = p[1];
                     dst[2] = p[2];
+                    if (vstride == 6) uw_vertex_color(p[2], uw_edge_b(e), float(extent), dst + 3);
                 }
             } else {
                 int32_t qx, qy, qz;
@@ -369,7 +372,7 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
     if (!ctx) return UW_ERR_INVALID_ARG;
     if (int rc = uw_check_resolution(ctx, n)) return rc;
     if (nchunks < 0 || !out || !out->info || !out->totals || (nchunks > 0 && !coords) ||
-        (out->vert_cap > 0 && !out->verts) || (out->ind_cap > 0 && !out->inds))
+        (out->vert_cap > 0 && !out->verts) || (out->ind_cap > 0 && !out->inds) || out->vertex_format > 1)
         return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
     UW_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -390,7 +393,8 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
                        out->totals);
     const float step = float(f.extent) / float(n);
     hipLaunchKernelGGL(k_emit, dim3(uint32_t(nchunks)), dim3(kThreads), 0, st, f.iso, f.extent, ctx->d_tables, coords,
-                       n, step, s.dens, cs, s.voff, s.act, s.ioff, out->info, out->verts, out->inds);
+                       n, step, s.dens, cs, s.voff, s.act, s.ioff, out->info, out->verts, out->inds,
+                       out->vertex_format == UW_VERTEX_XYZ_RGB ? 6u : 3u);
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
 }
@@ -420,7 +424,8 @@ int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t
     }
     const int32_t c[3] = {cx, cy, cz};
     UW_HIP(ctx, hipMemcpy(ctx->d_coord1, c, sizeof(c), hipMemcpyHostToDevice));
-    uw_mesh_out out = {ctx->d_verts1, ctx->d_inds1, ctx->d_info1, ctx->d_totals1, ctx->verts1_cap, ctx->inds1_cap};
+    uw_mesh_out out = {ctx->d_verts1, ctx->d_inds1, ctx->d_info1, ctx->d_totals1, ctx->verts1_cap, ctx->inds1_cap,
+                       UW_VERTEX_XYZ, 0u};
     if (int rc = uw_build_chunks_plain(ctx, ctx->d_coord1, 1, n, &out, nullptr, nullptr)) return rc;
     UW_HIP(ctx, hipMemcpy(info, ctx->d_info1, sizeof(uw_chunk_info), hipMemcpyDeviceToHost));
     if (info->n_verts > vert_cap || info->n_inds > ind_cap)

@@ -174,13 +174,15 @@ class Chunk:
 
 INFO_DTYPE = np.dtype([("flags", "<u4"), ("n_verts", "<u4"), ("n_inds", "<u4"), ("n_active", "<u4"),
                        ("vert_off", "<u8"), ("ind_off", "<u8")])
+VERTEX_XYZ = 0      # 3 floats per vertex: world position
+VERTEX_XYZ_RGB = 1  # 6 floats: position + sRGB colour, the reference's VertColor (draw.rs:4-9)
 TOTALS_DTYPE = np.dtype([("n_verts", "<u8"), ("n_inds", "<u8"), ("n_nonblank", "<u4"), ("overflow", "<u4")])
 
 
 @dataclass
 class MeshBatch:
     """Output of a multi-chunk build: device tensors plus host copies of the small records."""
-    verts: "torch.Tensor"   # (vert_cap, 3) f32
+    verts: "torch.Tensor"   # (vert_cap, 3) f32 xyz, or (vert_cap, 6) xyz + sRGB colour
     inds: "torch.Tensor"    # (ind_cap,) i32 (u32 bits)
     info_raw: "torch.Tensor"  # (nchunks * 32,) u8
     totals_raw: "torch.Tensor"  # (24,) u8
@@ -193,7 +195,7 @@ class MeshBatch:
         return self.totals_raw.cpu().numpy().view(TOTALS_DTYPE)[0]
 
     def chunk_mesh(self, i: int, info=None):
-        """(verts (V,3) f32, inds (I,) u32) of chunk i, as numpy arrays."""
+        """(verts (V,3) or (V,6) f32, inds (I,) u32) of chunk i, as numpy arrays."""
         r = (info if info is not None else self.info())[i]
         v0, nv, i0, ni = int(r["vert_off"]), int(r["n_verts"]), int(r["ind_off"]), int(r["n_inds"])
         v = self.verts[v0:v0 + nv].cpu().numpy()
@@ -201,9 +203,9 @@ class MeshBatch:
         return v, ix
 
 
-def _alloc_out(nchunks: int, vert_cap: int, ind_cap: int, device):
+def _alloc_out(nchunks: int, vert_cap: int, ind_cap: int, device, colors: bool = False):
     import torch
-    verts = torch.empty((max(vert_cap, 1), 3), dtype=torch.float32, device=device)
+    verts = torch.empty((max(vert_cap, 1), 6 if colors else 3), dtype=torch.float32, device=device)
     inds = torch.empty((max(ind_cap, 1),), dtype=torch.int32, device=device)
     info = torch.empty((max(nchunks, 1) * 32,), dtype=torch.uint8, device=device)
     totals = torch.empty((24,), dtype=torch.uint8, device=device)
@@ -211,7 +213,9 @@ def _alloc_out(nchunks: int, vert_cap: int, ind_cap: int, device):
 
 
 def _mesh_out(verts, inds, info, totals, vert_cap, ind_cap):
-    return _lib.UwMeshOut(verts.data_ptr(), inds.data_ptr(), info.data_ptr(), totals.data_ptr(), vert_cap, ind_cap)
+    fmt = VERTEX_XYZ_RGB if verts.shape[1] == 6 else VERTEX_XYZ
+    return _lib.UwMeshOut(verts.data_ptr(), inds.data_ptr(), info.data_ptr(), totals.data_ptr(), vert_cap, ind_cap,
+                          fmt, 0)
 
 
 def _coords_tensor(coords, device):
@@ -224,7 +228,7 @@ def _coords_tensor(coords, device):
 
 
 def build_chunks_plain(perlin: Perlin, coords, n: int, vert_cap: int | None = None, ind_cap: int | None = None,
-                       want_cases: bool = False, stream=None) -> MeshBatch:
+                       want_cases: bool = False, stream=None, colors: bool = False) -> MeshBatch:
     import torch
     dev = torch.device("cuda", perlin.device)
     ct = _coords_tensor(coords, dev)
@@ -234,7 +238,7 @@ def build_chunks_plain(perlin: Perlin, coords, n: int, vert_cap: int | None = No
         vert_cap = int(L.uw_max_verts(n)) * nch
     if ind_cap is None:
         ind_cap = int(L.uw_max_inds(n)) * nch
-    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev)
+    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev, colors)
     cases = torch.empty((max(nch, 1) * n ** 3,), dtype=torch.uint8, device=dev) if want_cases else None
     out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
     rc = L.uw_build_chunks_plain(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), nch, n, ctypes.byref(out),
@@ -259,8 +263,9 @@ def batch_capacity(res) -> tuple[int, int]:
 
 
 def build_chunks_batched(perlin: Perlin, coords, res, vert_cap: int | None = None, ind_cap: int | None = None,
-                         stream=None) -> MeshBatch:
-    """One persistent launch over a mixed-resolution chunk set (res: cells per axis per chunk)."""
+                         stream=None, colors: bool = False) -> MeshBatch:
+    """One persistent launch over a mixed-resolution chunk set (res: cells per axis per chunk).
+    colors=True writes the reference's VertColor layout (position + sRGB colour per vertex)."""
     import torch
     dev = torch.device("cuda", perlin.device)
     ct = _coords_tensor(coords, dev)
@@ -270,7 +275,7 @@ def build_chunks_batched(perlin: Perlin, coords, res, vert_cap: int | None = Non
         vc, ic = batch_capacity(rt[:nch].cpu().numpy() if nch else [])
         vert_cap = vc if vert_cap is None else vert_cap
         ind_cap = ic if ind_cap is None else ind_cap
-    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev)
+    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev, colors)
     out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
     rc = _lib.lib().uw_build_chunks_batched(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), ctypes.c_void_p(rt.data_ptr()),
                                             nch, ctypes.byref(out), _stream_ptr(stream))
@@ -281,7 +286,8 @@ def build_chunks_batched(perlin: Perlin, coords, res, vert_cap: int | None = Non
 class BatchedGraph:
     """The batched build captured once over fixed buffers and replayed (uw_batched_graph_create)."""
 
-    def __init__(self, perlin: Perlin, coords, res, vert_cap: int | None = None, ind_cap: int | None = None):
+    def __init__(self, perlin: Perlin, coords, res, vert_cap: int | None = None, ind_cap: int | None = None,
+                 colors: bool = False):
         import torch
         dev = torch.device("cuda", perlin.device)
         self.perlin = perlin
@@ -292,7 +298,7 @@ class BatchedGraph:
             vc, ic = batch_capacity(self.res[:nch].cpu().numpy())
             vert_cap = vc if vert_cap is None else vert_cap
             ind_cap = ic if ind_cap is None else ind_cap
-        verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev)
+        verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev, colors)
         self.batch = MeshBatch(verts, inds, info[: nch * 32], totals, None)
         self._out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
         self._g = ctypes.c_void_p()
@@ -323,16 +329,29 @@ class BatchedGraph:
 def get_batch_tuning(perlin: Perlin) -> dict:
     t = _lib.UwBatchTuning()
     _lib.check(_lib.lib().uw_get_batch_tuning(perlin.ctx, ctypes.byref(t)), perlin.ctx)
-    return {"block": t.block, "grid": t.grid, "slab": [int(v) for v in t.slab]}
+    return {"block": t.block, "grid": t.grid, "max_res": t.max_res, "slab": [int(v) for v in t.slab]}
 
 
 def set_batch_tuning(perlin: Perlin, tuning: dict):
     t = _lib.UwBatchTuning()
     t.block = int(tuning["block"])
     t.grid = int(tuning.get("grid", 0))
+    t.max_res = int(tuning.get("max_res", _lib.BATCH_MAX_RESOLUTION))
     for i, v in enumerate(tuning["slab"]):
         t.slab[i] = int(v)
     _lib.check(_lib.lib().uw_set_batch_tuning(perlin.ctx, ctypes.byref(t)), perlin.ctx)
+
+
+def density_at(perlin: Perlin, points, stream=None):
+    """iso_at at arbitrary noise-space points (N, 3) float64 -> (N,) float32 tensor."""
+    import torch
+    dev = torch.device("cuda", perlin.device)
+    pts = torch.as_tensor(np.ascontiguousarray(points, np.float64).reshape(-1, 3), device=dev)
+    out = torch.empty((max(pts.shape[0], 1),), dtype=torch.float32, device=dev)
+    rc = _lib.lib().uw_density_at(perlin.ctx, ctypes.c_void_p(pts.data_ptr()), pts.shape[0],
+                                  ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream))
+    _lib.check(rc, perlin.ctx)
+    return out[: pts.shape[0]]
 
 
 def sample_density(perlin: Perlin, coords, n: int, stream=None):
