@@ -71,6 +71,8 @@ enum uw_chunk_flags {
     UW_CHUNK_EARLY_SKIP = 2, /* every sample above iso: early_blank_check (chunk.rs:131-133)  */
     UW_CHUNK_OVERFLOW = 4,   /* did not fit the output capacity; nothing written for it       */
     UW_CHUNK_INVALID = 8,    /* resolution outside the accepted range; nothing built          */
+    UW_CHUNK_INDEX_WRAP = 16, /* u16 indices and more than 65535 vertices: the indices wrapped,
+                                 exactly as the reference's `ind as u16` (chunk.rs:243)        */
 };
 
 typedef struct uw_chunk_info {
@@ -95,9 +97,21 @@ typedef struct uw_totals {
  * chunk.rs:215-222 (taken, like the reference, from the edge's first-seen cell). */
 enum uw_vertex_format { UW_VERTEX_XYZ = 0, UW_VERTEX_XYZ_RGB = 1 };
 
+/* Index formats. UW_INDEX_U16 is the reference's wire format (chunk.rs:243, drawn as
+ * IndexFormat::Uint16 at state.rs:507): inds then points to uint16_t values. */
+enum uw_index_format { UW_INDEX_U32 = 0, UW_INDEX_U16 = 1 };
+
 /* Output description for the multi-chunk device entry points. All pointers are device
  * pointers: verts holds 3 (or 6, see vertex_format) floats per vertex (vert_cap vertices), inds
- * ind_cap u32 values, info one record per chunk, totals one record. */
+ * ind_cap u32 (or u16, see index_format) values, info one record per chunk, totals one record.
+ *
+ * Collision triangles (optional, NULL = not written; Build.tris of chunk.rs:42,167-175,245-250):
+ * triangle t of chunk c (indices ind_off + 3t .. + 2) is record ind_off / 3 + t of
+ *   tri_normals  3 floats: normalize((v1 - v0) x (v2 - v0)), or the raw cross product when its
+ *                length is 0 (util::Tri::new, util.rs:12-20);
+ *   tri_cells    the chunk-local cell (x n + y) n + z that emitted it (the Build.tris key). Within
+ *                a chunk the values are non-decreasing, so they are also the CSR of the
+ *                reference's cell -> triangles map (see uw_tris_around). */
 typedef struct uw_mesh_out {
     float* verts;
     uint32_t* inds;
@@ -106,7 +120,9 @@ typedef struct uw_mesh_out {
     uint64_t vert_cap;
     uint64_t ind_cap;
     uint32_t vertex_format; /* enum uw_vertex_format */
-    uint32_t reserved;
+    uint32_t index_format;  /* enum uw_index_format  */
+    float* tri_normals;
+    uint32_t* tri_cells;
 } uw_mesh_out;
 
 typedef struct uw_ctx uw_ctx;
@@ -162,6 +178,35 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
  * UW_ERR_OVERFLOW nothing is copied and info->n_verts / n_inds give the sizes needed. */
 int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, float* verts,
                         uint64_t vert_cap, uint32_t* inds, uint64_t ind_cap, uw_chunk_info* info);
+
+/* Host output of one chunk with every wire option (the fields mean what they mean in
+ * uw_mesh_out, for host memory): verts vert_cap vertices of 3 or 6 floats, inds ind_cap u32 or
+ * u16 values, tri_normals / tri_cells (optional) ind_cap / 3 records. */
+typedef struct uw_host_out {
+    float* verts;
+    uint64_t vert_cap;
+    void* inds;
+    uint64_t ind_cap;
+    uint32_t vertex_format; /* enum uw_vertex_format */
+    uint32_t index_format;  /* enum uw_index_format  */
+    float* tri_normals;
+    uint32_t* tri_cells;
+} uw_host_out;
+
+/* uw_chunk_build_host with the reference's full output: VertColor vertices (draw.rs:4-9), u16
+ * indices (chunk.rs:243) and the collision triangles of Build.tris (chunk.rs:167-175,245-250). */
+int uw_chunk_build_host_out(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, const uw_host_out* out,
+                            uw_chunk_info* info);
+
+/* Chunk::tris_around (chunk.rs:315-342), host-only: the triangles of the cells within `range`
+ * cells of the point local_pos_percent (chunk-local position / chunk extent, per axis) of a chunk
+ * of n cells per axis, given that chunk's tri_cells (ntris records). Cells are visited x -> y -> z
+ * over [max(m - range, 0), min(m + range, n)] with m = floor(percent * n), and each cell's
+ * triangles in emission order, so ids (chunk-local triangle numbers) come out in the reference's
+ * order. A box whose end lies below its start is empty (the reference's loop would wrap there).
+ * Writes at most cap ids; *count receives the full number (UW_ERR_OVERFLOW if it exceeds cap). */
+int uw_tris_around(const uint32_t* tri_cells, uint32_t ntris, int32_t n, float px, float py, float pz,
+                   int32_t range, uint32_t* ids, uint32_t cap, uint32_t* count);
 
 /* ---- Batched path: one persistent launch over many chunks of mixed resolution ------------- */
 

@@ -10,13 +10,16 @@
 //   Chunk::verts_buffer_slice()      Chunk::verts_buffer_slice              chunk.rs:346
 //   Chunk::inds_buffer_slice()       Chunk::inds_buffer_slice               chunk.rs:347
 //   Chunk::num_inds()                Chunk::num_inds                        chunk.rs:348
+//   Chunk::tris_around(pct, range)   Chunk::tris_around -> Vec<Tri>         chunk.rs:315-342
+//   uw::Tri                          util::Tri {verts, normal}              util.rs:7-20
 //   uw::ChunkBatch::build(...)       World::build_full_step / build_step    world.rs:113-145
 //                                    for a whole candidate set in one launch
 //
 // Differences, by design: the build finishes in one call (build_partial returns true); vertices
-// are xyz f32 world positions (the reference's VertColor carries a colour too, draw.rs:4-9);
-// indices are u32 (the reference truncates to u16, chunk.rs:243). The reference panics when a
-// blank chunk's buffers are requested (chunk.rs:345-347); this facade throws uw::Error.
+// are xyz f32 world positions unless the chunk is made with Format::VertColor (the reference's
+// 6-float VertColor, draw.rs:4-9); indices are u32 unless made with u16 = true (the reference's
+// `ind as u16`, chunk.rs:243). The reference panics when a blank chunk's buffers are requested
+// (chunk.rs:345-347); this facade throws uw::Error.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -77,11 +80,20 @@ private:
     uw_params params_{};
 };
 
+// Collision triangle: world-space vertices and unit normal (util.rs:7-20).
+struct Tri {
+    float verts[3][3];
+    float normal[3];
+};
+
 class Chunk {
 public:
     static constexpr int32_t INTERNAL_SIZE = 12;  // chunk.rs:6
+    enum class Format { Xyz, VertColor };
 
-    Chunk(int32_t x, int32_t y, int32_t z, int32_t resolution = INTERNAL_SIZE) : pos_{x, y, z}, n_(resolution) {}
+    Chunk(int32_t x, int32_t y, int32_t z, int32_t resolution = INTERNAL_SIZE, Format format = Format::Xyz,
+          bool u16 = false)
+        : pos_{x, y, z}, n_(resolution), fmt_(format), u16_(u16) {}
 
     void build_full(const Perlin& perlin) {
         while (!build_partial(perlin)) {
@@ -92,17 +104,59 @@ public:
     // reference's state machine.
     bool build_partial(const Perlin& perlin) {
         if (done_) return true;
-        verts_.resize(size_t(uw_max_verts(n_)) * 3);
-        inds_.resize(size_t(uw_max_inds(n_)));
-        check(uw_chunk_build_host(perlin.ctx(), pos_[0], pos_[1], pos_[2], n_, verts_.data(), verts_.size() / 3,
-                                  inds_.data(), inds_.size(), &info_),
-              perlin.ctx(), "uw_chunk_build_host");
-        verts_.resize(size_t(info_.n_verts) * 3);
-        inds_.resize(size_t(info_.n_inds));
+        const size_t vfl = floats_per_vertex();
+        const size_t icap = size_t(uw_max_inds(n_));
+        verts_.resize(size_t(uw_max_verts(n_)) * vfl);
+        if (u16_) inds16_.resize(icap);
+        else inds_.resize(icap);
+        normals_.resize(icap);  // 3 floats per triangle
+        tri_cells_.resize(icap / 3);
+        uw_host_out out = {verts_.data(),
+                           verts_.size() / vfl,
+                           u16_ ? static_cast<void*>(inds16_.data()) : static_cast<void*>(inds_.data()),
+                           icap,
+                           fmt_ == Format::VertColor ? uint32_t(UW_VERTEX_XYZ_RGB) : uint32_t(UW_VERTEX_XYZ),
+                           u16_ ? uint32_t(UW_INDEX_U16) : uint32_t(UW_INDEX_U32),
+                           normals_.data(),
+                           tri_cells_.data()};
+        check(uw_chunk_build_host_out(perlin.ctx(), pos_[0], pos_[1], pos_[2], n_, &out, &info_), perlin.ctx(),
+              "uw_chunk_build_host_out");
+        verts_.resize(size_t(info_.n_verts) * vfl);
+        if (u16_) inds16_.resize(size_t(info_.n_inds));
+        else inds_.resize(size_t(info_.n_inds));
+        normals_.resize(size_t(info_.n_inds));
+        tri_cells_.resize(size_t(info_.n_inds) / 3);
         verts_.shrink_to_fit();
         inds_.shrink_to_fit();
+        inds16_.shrink_to_fit();
+        normals_.shrink_to_fit();
+        tri_cells_.shrink_to_fit();
         done_ = true;
         return true;
+    }
+
+    // Collision triangles of the cells within `range` cells of the chunk-local position
+    // local_pos_percent (position / chunk extent), in the reference's order (chunk.rs:315-342).
+    std::vector<Tri> tris_around(const float local_pos_percent[3], int32_t range) const {
+        std::vector<Tri> tris;
+        if (!done_ || tri_cells_.empty()) return tris;
+        std::vector<uint32_t> ids(tri_cells_.size());
+        uint32_t count = 0;
+        check(uw_tris_around(tri_cells_.data(), uint32_t(tri_cells_.size()), n_, local_pos_percent[0],
+                             local_pos_percent[1], local_pos_percent[2], range, ids.data(), uint32_t(ids.size()),
+                             &count),
+              nullptr, "uw_tris_around");
+        const size_t vfl = floats_per_vertex();
+        tris.resize(count);
+        for (uint32_t k = 0; k < count; ++k) {
+            const uint32_t t = ids[k];
+            for (int c = 0; c < 3; ++c) {
+                const size_t v = u16_ ? inds16_[3 * t + c] : inds_[3 * t + c];
+                for (int d = 0; d < 3; ++d) tris[k].verts[c][d] = verts_[v * vfl + d];
+            }
+            for (int d = 0; d < 3; ++d) tris[k].normal[d] = normals_[3 * t + d];
+        }
+        return tris;
     }
 
     bool not_blank() const { return done_ && info_.n_inds > 0; }
@@ -113,19 +167,34 @@ public:
     }
     const std::vector<uint32_t>& inds_buffer_slice() const {
         if (!not_blank()) throw Error(UW_ERR_INVALID_ARG, "inds_buffer_slice() on a blank chunk");
+        if (u16_) throw Error(UW_ERR_INVALID_ARG, "inds_buffer_slice() on a u16 chunk: use inds16_buffer_slice()");
         return inds_;
     }
+    const std::vector<uint16_t>& inds16_buffer_slice() const {
+        if (!not_blank()) throw Error(UW_ERR_INVALID_ARG, "inds16_buffer_slice() on a blank chunk");
+        if (!u16_) throw Error(UW_ERR_INVALID_ARG, "inds16_buffer_slice() on a u32 chunk");
+        return inds16_;
+    }
+    const std::vector<float>& tri_normals() const { return normals_; }
+    const std::vector<uint32_t>& tri_cells() const { return tri_cells_; }
     size_t num_inds() const { return info_.n_inds; }
     size_t num_verts() const { return info_.n_verts; }
     const uw_chunk_info& info() const { return info_; }
 
 private:
+    size_t floats_per_vertex() const { return fmt_ == Format::VertColor ? 6 : 3; }
+
     int32_t pos_[3];
     int32_t n_;
+    Format fmt_;
+    bool u16_;
     bool done_ = false;
     uw_chunk_info info_{};
     std::vector<float> verts_;
     std::vector<uint32_t> inds_;
+    std::vector<uint16_t> inds16_;
+    std::vector<float> normals_;
+    std::vector<uint32_t> tri_cells_;
 };
 
 // A whole chunk set built by one persistent launch (uw_build_chunks_batched); results copied back
@@ -155,7 +224,8 @@ public:
         reserve(n, vcap, icap);
         hip(hipMemcpy(d_coords_, coords.data(), coords.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         hip(hipMemcpy(d_res_, res.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
-        uw_mesh_out out = {d_verts_, d_inds_, d_info_, d_totals_, vcap, icap, UW_VERTEX_XYZ, 0u};
+        uw_mesh_out out = {d_verts_, d_inds_, d_info_, d_totals_, vcap, icap, UW_VERTEX_XYZ, UW_INDEX_U32, nullptr,
+                           nullptr};
         check(uw_build_chunks_batched(perlin_.ctx(), d_coords_, d_res_, int32_t(n), &out, nullptr), perlin_.ctx(),
               "uw_build_chunks_batched");
         hip(hipDeviceSynchronize());

@@ -83,6 +83,16 @@ def lib():
                                        P(ctypes.c_float), P(ctypes.c_float), ctypes.c_int64, P(ctypes.c_uint32),
                                        ctypes.c_int64, P(UwoMeshCounts)]
         L.uwo_mesh_ref_rgb.restype = ctypes.c_int
+        L.uwo_mesh_ref_tris.argtypes = [P(UwoParams), P(ctypes.c_int32), ctypes.c_int32, P(ctypes.c_float),
+                                        P(ctypes.c_float), ctypes.c_int64, P(ctypes.c_uint32), ctypes.c_int64,
+                                        P(ctypes.c_uint32), P(ctypes.c_float), P(UwoMeshCounts)]
+        L.uwo_mesh_ref_tris.restype = ctypes.c_int
+        L.uwo_tri_normal.argtypes = [P(ctypes.c_float)] * 4
+        L.uwo_tri_normal.restype = None
+        L.uwo_tris_around.argtypes = [P(ctypes.c_uint32), ctypes.c_int64, ctypes.c_int32, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_int32, P(ctypes.c_uint32),
+                                      ctypes.c_int64]
+        L.uwo_tris_around.restype = ctypes.c_int64
         L.uwo_mesh_fp64.argtypes = [P(UwoParams), P(ctypes.c_int32), ctypes.c_int32, P(ctypes.c_double),
                                     P(ctypes.c_double), ctypes.c_int64, P(ctypes.c_uint32), ctypes.c_int64,
                                     P(ctypes.c_uint8), P(UwoMeshCounts)]
@@ -225,6 +235,44 @@ def mesh_ref_rgb(params: dict, coord, n: int, dens: np.ndarray):
     if rc != 0:
         raise RuntimeError("uwo_mesh_ref_rgb failed: %d" % rc)
     return verts[:cnt.n_verts].copy(), colors[:cnt.n_verts].copy(), inds[:cnt.n_inds].copy()
+
+
+def mesh_ref_tris(params: dict, coord, n: int, dens: np.ndarray):
+    """mesh_ref with Build.tris: (verts, inds, tri_cells (T,) u32, tri_normals (T,3) f32)."""
+    s = make_params(params)
+    c = np.ascontiguousarray(coord, np.int32)
+    d = np.ascontiguousarray(dens, np.float32)
+    vcap, icap = vert_capacity(n), 15 * n ** 3 + 16
+    verts = np.zeros((vcap, 3), np.float32)
+    inds = np.zeros(icap, np.uint32)
+    cells = np.zeros(icap // 3 + 1, np.uint32)
+    normals = np.zeros((icap // 3 + 1, 3), np.float32)
+    cnt = UwoMeshCounts()
+    rc = lib().uwo_mesh_ref_tris(ctypes.byref(s), _ptr(c, ctypes.c_int32), n, _ptr(d, ctypes.c_float),
+                                 _ptr(verts, ctypes.c_float), vcap, _ptr(inds, ctypes.c_uint32), icap,
+                                 _ptr(cells, ctypes.c_uint32), _ptr(normals, ctypes.c_float), ctypes.byref(cnt))
+    if rc != 0:
+        raise RuntimeError("uwo_mesh_ref_tris failed: %d" % rc)
+    nt = cnt.n_inds // 3
+    return verts[:cnt.n_verts].copy(), inds[:cnt.n_inds].copy(), cells[:nt].copy(), normals[:nt].copy()
+
+
+def tri_normal(a, b, c) -> np.ndarray:
+    """util::Tri::new's normal of one triangle (f32)."""
+    pts = [np.ascontiguousarray(v, np.float32) for v in (a, b, c)]
+    out = np.zeros(3, np.float32)
+    lib().uwo_tri_normal(*[_ptr(v, ctypes.c_float) for v in pts], _ptr(out, ctypes.c_float))
+    return out
+
+
+def tris_around(tri_cells: np.ndarray, n: int, pct, range_: int) -> np.ndarray:
+    """Chunk::tris_around restated literally: triangle ids in the reference's order."""
+    tc = np.ascontiguousarray(tri_cells, np.uint32)
+    cap = max(len(tc), 1)
+    ids = np.zeros(cap, np.uint32)
+    k = lib().uwo_tris_around(_ptr(tc, ctypes.c_uint32), len(tc), int(n), float(pct[0]), float(pct[1]),
+                              float(pct[2]), int(range_), _ptr(ids, ctypes.c_uint32), cap)
+    return ids[:k].copy()
 
 
 def vertex_color(world_z: float, corner_b: int, chunk_size: float = 16.0) -> np.ndarray:

@@ -356,9 +356,54 @@ void uwo_vertex_color(float world_z, int corner_b_idx, float chunk_size, float r
     for (int k = 0; k < 3; ++k) rgb[k] = powf((lin[k] / 255.0f + 0.055f) / 1.055f, 2.4f);
 }
 
+/*
+ * Collision-triangle normal, util::Tri::new (util.rs:12-20) with cgmath 0.18's Vector3 (crate not
+ * vendored; its cross, dot-as-sum-of-products and magnitude = sqrt(dot) are recalled):
+ * n = (b - a) x (c - a), then n / |n| unless |n| == 0 (safe_normalize, util.rs:60-63).
+ */
+void uwo_tri_normal(const float* a, const float* b, const float* c, float* out) {
+    const float u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const float v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    const float n[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2], u[0] * v[1] - u[1] * v[0]};
+    const float mag = sqrtf(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int k = 0; k < 3; ++k) out[k] = mag == 0.0f ? n[k] : n[k] / mag;
+}
+
+/*
+ * Chunk::tris_around (chunk.rs:315-342), literally: for x, y, z over the clamped box, append the
+ * triangles of the map entry (x, y, z). The map is the per-triangle cell list tri_cells (the
+ * Build.tris key of each triangle, chunk.rs:173); entries are looked up by a scan. A box whose end
+ * lies below its start is empty. Returns the number of ids (writes at most cap).
+ */
+int64_t uwo_tris_around(const uint32_t* tri_cells, int64_t ntris, int32_t n, float px, float py, float pz,
+                        int32_t range, uint32_t* ids, int64_t cap) {
+    const float pct[3] = {px, py, pz};
+    int64_t lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+        float f = floorf(pct[k] * (float)n);
+        int64_t m = isnan(f) ? 0 : f >= 2147483647.0f ? 2147483647 : f <= -2147483648.0f ? -2147483648LL : (int64_t)f;
+        lo[k] = m - range < 0 ? 0 : m - range;
+        hi[k] = m + range > n ? n : m + range;
+    }
+    int64_t cnt = 0;
+    for (int64_t x = lo[0]; x <= hi[0]; ++x)
+        for (int64_t y = lo[1]; y <= hi[1]; ++y)
+            for (int64_t z = lo[2]; z <= hi[2]; ++z) {
+                if (x >= n || y >= n || z >= n) continue; /* no map entry for cell index n */
+                const uint32_t key = (uint32_t)((x * n + y) * n + z);
+                for (int64_t t = 0; t < ntris; ++t)
+                    if (tri_cells[t] == key) {
+                        if (cnt < cap) ids[cnt] = (uint32_t)t;
+                        ++cnt;
+                    }
+            }
+    return cnt;
+}
+
 static int mesh_ref_impl(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
                          int dedup_mode, float* verts, float* colors, int64_t vcap, uint32_t* inds,
-                         int64_t icap, uint8_t* cases, uint32_t* tri_cells, uwo_mesh_counts* counts);
+                         int64_t icap, uint8_t* cases, uint32_t* tri_cells, float* tri_normals,
+                         uwo_mesh_counts* counts);
 
 /*
  * Reference-faithful mesher. dedup_mode 0 = the reference's literal linear search over the
@@ -374,7 +419,7 @@ int uwo_mesh_ref(const uwo_params* prm, const int32_t* coord, int32_t n, const f
                  int dedup_mode, float* verts, int64_t vcap, uint32_t* inds, int64_t icap,
                  uint8_t* cases, uint32_t* tri_cells, uwo_mesh_counts* counts) {
     return mesh_ref_impl(prm, coord, n, dens, dedup_mode, verts, NULL, vcap, inds, icap, cases, tri_cells,
-                         counts);
+                         NULL, counts);
 }
 
 /* uwo_mesh_ref plus colors: 3 floats per vertex, the VertColor colour of the vertex's first-seen
@@ -382,12 +427,22 @@ int uwo_mesh_ref(const uwo_params* prm, const int32_t* coord, int32_t n, const f
 int uwo_mesh_ref_rgb(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
                      float* verts, float* colors, int64_t vcap, uint32_t* inds, int64_t icap,
                      uwo_mesh_counts* counts) {
-    return mesh_ref_impl(prm, coord, n, dens, 1, verts, colors, vcap, inds, icap, NULL, NULL, counts);
+    return mesh_ref_impl(prm, coord, n, dens, 1, verts, colors, vcap, inds, icap, NULL, NULL, NULL, counts);
+}
+
+/* uwo_mesh_ref plus the collision triangles of Build.tris (chunk.rs:167-175,245-250): per
+ * triangle its cell key and the normal of util::Tri::new over the three emitted positions. */
+int uwo_mesh_ref_tris(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
+                      float* verts, int64_t vcap, uint32_t* inds, int64_t icap, uint32_t* tri_cells,
+                      float* tri_normals, uwo_mesh_counts* counts) {
+    return mesh_ref_impl(prm, coord, n, dens, 1, verts, NULL, vcap, inds, icap, NULL, tri_cells, tri_normals,
+                         counts);
 }
 
 static int mesh_ref_impl(const uwo_params* prm, const int32_t* coord, int32_t n, const float* dens,
                          int dedup_mode, float* verts, float* colors, int64_t vcap, uint32_t* inds,
-                         int64_t icap, uint8_t* cases, uint32_t* tri_cells, uwo_mesh_counts* counts) {
+                         int64_t icap, uint8_t* cases, uint32_t* tri_cells, float* tri_normals,
+                         uwo_mesh_counts* counts) {
     const int32_t L = n + 1;
     const float scale = uwo_step(prm, n);
     const float iso = prm->iso;
@@ -421,6 +476,7 @@ static int mesh_ref_impl(const uwo_params* prm, const int32_t* coord, int32_t n,
                 if (cs != 0 && cs != 255) ++nact;
                 uint64_t row = UW_MC_TRI_PACKED[cs];
                 int ntri = (int)(row & 15u);
+                float cur_tri[3][3];
                 for (int j = 0; j < 3 * ntri; ++j) {
                     int e = (int)((row >> (4 + 4 * j)) & 15u);
                     int ia = UW_MC_EDGE_A_IDX[e], ib = UW_MC_EDGE_B_IDX[e];
@@ -472,6 +528,10 @@ static int mesh_ref_impl(const uwo_params* prm, const int32_t* coord, int32_t n,
                     }
                     if (ni < icap && inds) inds[ni] = (uint32_t)id;
                     if (tri_cells && j % 3 == 0 && ni / 3 < icap / 3) tri_cells[ni / 3] = (uint32_t)cell;
+                    /* current_tri holds this emission's vert.pos (chunk.rs:224-231,245-249) */
+                    for (int k = 0; k < 3; ++k) cur_tri[j % 3][k] = m[k] + off[k];
+                    if (tri_normals && j % 3 == 2 && ni / 3 < icap / 3)
+                        uwo_tri_normal(cur_tri[0], cur_tri[1], cur_tri[2], tri_normals + 3 * (ni / 3));
                     ++ni;
                 }
             }

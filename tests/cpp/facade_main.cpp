@@ -2,6 +2,8 @@
 // chunk.rs: Perlin::new(seed), Chunk::new(pos), build_full, not_blank, buffer slices, num_inds;
 // then the same chunks through ChunkBatch. Prints one line per chunk:
 //   <path> <x> <y> <z> <not_blank> <num_verts> <num_inds> <fnv1a of vertex and index bytes>
+// and, for each non-blank chunk, the collision query of boid.rs:205 at the chunk's centre:
+//   tris <x> <y> <z> <count> <fnv1a of the returned Tri records>
 // tests/test_gpu_facade.py compares the lines with the Python binding's results.
 #include <cstdio>
 #include <cstring>
@@ -47,6 +49,13 @@ int main(int argc, char** argv) {
             }
             printf("chunk %d %d %d %d %zu %zu %016llx\n", coords[i][0], coords[i][1], coords[i][2], int(ch.not_blank()),
                    ch.num_verts(), ch.num_inds(), (unsigned long long)h);
+            if (ch.not_blank()) {
+                const float pct[3] = {0.5f, 0.5f, 0.5f};
+                const std::vector<uw::Tri> tris = ch.tris_around(pct, 3);
+                const uint64_t th = tris.empty() ? 0 : fnv1a(tris.data(), tris.size() * sizeof(uw::Tri));
+                printf("tris %d %d %d %zu %016llx\n", coords[i][0], coords[i][1], coords[i][2], tris.size(),
+                       (unsigned long long)th);
+            }
             flat.insert(flat.end(), coords[i], coords[i] + 3);
             res.push_back(12);
         }

@@ -27,6 +27,8 @@ def test_cpp_facade_matches_python_binding():
     lines = [l.split() for l in r.stdout.strip().splitlines()]
     perlin = Perlin(None, seed=7)
     want = {}
+    tri_lines = [l for l in lines if l[0] == "tris"]
+    lines = [l for l in lines if l[0] != "tris"]
     for path, x, y, z, nb, nv, ni, h in lines:
         key = (int(x), int(y), int(z))
         if key not in want:
@@ -41,3 +43,12 @@ def test_cpp_facade_matches_python_binding():
         if w[0]:
             assert int(nv) == w[1] and int(h, 16) == w[3], (path, key)
     assert any(v[0] for v in want.values())
+    assert len(tri_lines) == sum(v[0] for v in want.values())
+    for _, x, y, z, cnt, h in tri_lines:
+        ch = Chunk((int(x), int(y), int(z)))
+        ch.build_full(perlin)
+        tris = ch.tris_around((0.5, 0.5, 0.5), 3)
+        assert int(cnt) == len(tris)
+        if tris:
+            blob = np.concatenate([np.concatenate([t.verts.ravel(), t.normal]) for t in tris]).astype(np.float32)
+            assert int(h, 16) == fnv1a(blob.tobytes())

@@ -5,9 +5,10 @@ include/uwmesh.h (libuwmesh.so); this package is the host-side mirror of the ref
 interface (noise::Perlin + Chunk) and the device-resident multi-chunk forms.
 """
 from . import params
-from .engine import (BatchedGraph, Chunk, MeshBatch, Perlin, batch_capacity, build_chunks_batched,
+from .engine import (BatchedGraph, Chunk, MeshBatch, Perlin, Tri, batch_capacity, build_chunks_batched,
                      build_chunks_plain, density_at, get_batch_tuning, permutation_table, sample_density,
-                     set_batch_tuning)
+                     set_batch_tuning, tri_ids_around)
 
-__all__ = ["params", "Perlin", "Chunk", "MeshBatch", "BatchedGraph", "build_chunks_plain", "build_chunks_batched",
-           "batch_capacity", "sample_density", "density_at", "permutation_table", "get_batch_tuning", "set_batch_tuning"]
+__all__ = ["params", "Perlin", "Chunk", "Tri", "MeshBatch", "BatchedGraph", "build_chunks_plain",
+           "build_chunks_batched", "batch_capacity", "sample_density", "density_at", "permutation_table",
+           "get_batch_tuning", "set_batch_tuning", "tri_ids_around"]

@@ -24,6 +24,7 @@ UW_ERR_OOM = -5
 CHUNK_BLANK = 1
 CHUNK_EARLY_SKIP = 2
 CHUNK_OVERFLOW = 4
+CHUNK_INDEX_WRAP = 16
 
 P = ctypes.POINTER
 c_void_p = ctypes.c_void_p
@@ -68,7 +69,13 @@ class UwBatchTuning(ctypes.Structure):
 class UwMeshOut(ctypes.Structure):
     _fields_ = [("verts", c_void_p), ("inds", c_void_p), ("info", c_void_p), ("totals", c_void_p),
                 ("vert_cap", ctypes.c_uint64), ("ind_cap", ctypes.c_uint64), ("vertex_format", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32)]
+                ("index_format", ctypes.c_uint32), ("tri_normals", c_void_p), ("tri_cells", c_void_p)]
+
+
+class UwHostOut(ctypes.Structure):
+    _fields_ = [("verts", c_void_p), ("vert_cap", ctypes.c_uint64), ("inds", c_void_p), ("ind_cap", ctypes.c_uint64),
+                ("vertex_format", ctypes.c_uint32), ("index_format", ctypes.c_uint32), ("tri_normals", c_void_p),
+                ("tri_cells", c_void_p)]
 
 
 # name -> (restype, argtypes); every symbol include/uwmesh.h declares.
@@ -89,6 +96,11 @@ SIGNATURES = {
                                              c_void_p, c_void_p]),
     "uw_chunk_build_host": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            c_void_p, ctypes.c_uint64, c_void_p, ctypes.c_uint64, P(UwChunkInfo)]),
+    "uw_chunk_build_host_out": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                               ctypes.c_int32, P(UwHostOut), P(UwChunkInfo)]),
+    "uw_tris_around": (ctypes.c_int, [c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_int32, c_void_p, ctypes.c_uint32,
+                                      P(ctypes.c_uint32)]),
     "uw_get_batch_tuning": (ctypes.c_int, [c_void_p, P(UwBatchTuning)]),
     "uw_set_batch_tuning": (ctypes.c_int, [c_void_p, P(UwBatchTuning)]),
     "uw_build_chunks_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut), c_void_p]),

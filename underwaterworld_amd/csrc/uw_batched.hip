@@ -44,7 +44,7 @@ struct BatchArgs {
     float* verts;
     uint32_t* inds;
     uw_vertex_tape* tape;  // optional per-vertex record for the adjoint
-    uint32_t vstride;      // floats per vertex: 3 (xyz) or 6 (xyz + sRGB colour)
+    uw_emit_opts eo;       // vertex / index formats and optional collision records
     uint64_t vcap, icap;
     int32_t max_res;      // largest resolution accepted by this launch (sizes the LDS)
     int32_t list_cap;     // entries of the compacted active-cell list
@@ -168,22 +168,26 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
     const uint64_t row = TRI[cs];
     const int nidx = 3 * int(row & 15u);
     const uint64_t vb = vo + myv;
+    const uw_emit_opts& eo = a.eo;
+    const bool coll = eo.tri_normals || eo.tri_cells;
+    float q0[3] = {0.f, 0.f, 0.f}, q1[3] = {0.f, 0.f, 0.f};
     uint32_t seen = 0;
     for (int j = 0; j < nidx; ++j) {
         const uint32_t ed = uint32_t(row >> (4 + 4 * j)) & 15u;
         const uint32_t bit = 1u << ed;
+        const bool fresh = (owned & bit) && !(seen & bit);
+        float p[3];
+        if (fresh || coll) uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
         uint32_t id;
         if (owned & bit) {
             const uint32_t rank = __popc(uint32_t(tab->before[cs][ed]) & owned);
             id = myv + rank;
-            if (!(seen & bit)) {
-                float p[3];
-                uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
-                float* dst = a.verts + a.vstride * (vb + rank);
+            if (fresh) {
+                float* dst = a.verts + eo.vstride * (vb + rank);
                 dst[0] = p[0];
                 dst[1] = p[1];
                 dst[2] = p[2];
-                if (a.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
+                if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
                 if (a.tape) {
                     const uint32_t ia = uw_edge_a(ed), ib2 = uw_edge_b(ed);
                     const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib2);
@@ -206,7 +210,27 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
             id = owner_voff(q, qw) + __popc(uint32_t(tab->before[qw & 255u][qe]) & uw_owned_mask(qx, qy, qz));
         }
         seen |= bit;
-        a.inds[ib + j] = id;
+        uw_store_index(a.inds, eo.ind16, ib + j, id);
+        if (coll) {
+            const int k = j % 3;
+            if (k == 0) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) q0[d] = p[d];
+            } else if (k == 1) {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) q1[d] = p[d];
+            } else {
+                const uint64_t t = (ib + j) / 3;
+                if (eo.tri_normals) {
+                    float nm[3];
+                    uw_tri_normal(q0, q1, p, nm);
+                    eo.tri_normals[3 * t] = nm[0];
+                    eo.tri_normals[3 * t + 1] = nm[1];
+                    eo.tri_normals[3 * t + 2] = nm[2];
+                }
+                if (eo.tri_cells) eo.tri_cells[t] = uint32_t((x * n + y) * n + z);
+            }
+        }
     }
 }
 
@@ -357,6 +381,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                         atomicOr(&a.totals->overflow, 1u);
                     }
                 }
+                if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
                 uw_chunk_info r = {flags, nv, ni, na, vo, io};
                 a.info[c] = r;
                 s_vo = vo;
@@ -436,6 +461,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                     atomicOr(&a.totals->overflow, 1u);
                 }
             }
+            if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
             uw_chunk_info r = {flags, nv, ni, na, vo, io};
             a.info[c] = r;
             s_vo = vo;
@@ -624,7 +650,7 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.vcap = out->vert_cap;
     a.icap = out->ind_cap;
     a.tape = tape;
-    a.vstride = out->vertex_format == UW_VERTEX_XYZ_RGB ? 6u : 3u;
+    a.eo = uw_make_emit_opts(out);
     a.max_res = tune.max_res;
     a.list_cap = list_cap_for(tune);
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
@@ -640,7 +666,8 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
 }
 
 bool args_ok(int32_t nchunks, const int32_t* coords, const int32_t* res, const uw_mesh_out* out) {
-    return nchunks >= 0 && out && out->info && out->totals && out->vertex_format <= 1 && (nchunks == 0 || (coords && res)) &&
+    return nchunks >= 0 && out && out->info && out->totals && out->vertex_format <= 1 && out->index_format <= 1 &&
+           (nchunks == 0 || (coords && res)) &&
            (out->vert_cap == 0 || out->verts) && (out->ind_cap == 0 || out->inds);
 }
 

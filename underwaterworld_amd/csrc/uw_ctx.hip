@@ -225,6 +225,8 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(ctx->d_inds1);
     (void)hipFree(ctx->d_info1);
     (void)hipFree(ctx->d_totals1);
+    (void)hipFree(ctx->d_norm1);
+    (void)hipFree(ctx->d_cells1);
     (void)hipFree(ctx->d_batch_head);
     (void)hipFree(ctx->d_batch_order);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);

@@ -70,6 +70,8 @@ struct uw_ctx {
     uint32_t* d_inds1 = nullptr;
     uw_chunk_info* d_info1 = nullptr;
     uw_totals* d_totals1 = nullptr;
+    float* d_norm1 = nullptr;     // collision normals of the single-chunk build
+    uint32_t* d_cells1 = nullptr; // collision cells of the single-chunk build
     uint64_t verts1_cap = 0, inds1_cap = 0;
     std::string last_error;
 };

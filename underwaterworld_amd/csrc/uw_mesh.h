@@ -122,6 +122,43 @@ __device__ __forceinline__ void uw_vertex_color(float world_z, uint32_t corner_b
     for (int i = 0; i < 3; ++i) rgb[i] = powf((lin[i] / 255.0f + 0.055f) / 1.055f, 2.4f);
 }
 
+// Collision-triangle normal (util::Tri::new, util.rs:12-20, with cgmath's Vector3 cross, dot and
+// magnitude): n = (b - a) x (c - a); n / |n| unless |n| == 0, then n itself. f32, the reference's
+// operation order: |n| = sqrt((x x + y y) + z z).
+__device__ __forceinline__ void uw_tri_normal(const float a[3], const float b[3], const float c[3], float out[3]) {
+    const float ux = b[0] - a[0], uy = b[1] - a[1], uz = b[2] - a[2];
+    const float vx = c[0] - a[0], vy = c[1] - a[1], vz = c[2] - a[2];
+    const float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+    const float mag = sqrtf(nx * nx + ny * ny + nz * nz);
+    if (mag == 0.0f) {
+        out[0] = nx; out[1] = ny; out[2] = nz;
+    } else {
+        out[0] = nx / mag; out[1] = ny / mag; out[2] = nz / mag;
+    }
+}
+
+// Per-launch output options shared by the emit kernels.
+struct uw_emit_opts {
+    uint32_t vstride;     // floats per vertex: 3 (xyz) or 6 (xyz + sRGB colour)
+    uint32_t ind16;       // 1: indices are written as u16 (chunk.rs:243)
+    float* tri_normals;   // optional, 3 floats per triangle
+    uint32_t* tri_cells;  // optional, 1 u32 per triangle
+};
+
+__host__ inline uw_emit_opts uw_make_emit_opts(const uw_mesh_out* out) {
+    uw_emit_opts o;
+    o.vstride = out->vertex_format == UW_VERTEX_XYZ_RGB ? 6u : 3u;
+    o.ind16 = out->index_format == UW_INDEX_U16 ? 1u : 0u;
+    o.tri_normals = out->tri_normals;
+    o.tri_cells = out->tri_cells;
+    return o;
+}
+
+__device__ __forceinline__ void uw_store_index(uint32_t* inds, uint32_t ind16, uint64_t at, uint32_t id) {
+    if (ind16) reinterpret_cast<uint16_t*>(inds)[at] = uint16_t(id);
+    else inds[at] = id;
+}
+
 // World position of the lerp on edge e of cell (x, y, z) (chunk.rs:185-229), f32 with the
 // reference's operation order (the file is compiled with -ffp-contract=off).
 __device__ __forceinline__ void uw_edge_vertex(uint32_t e, int32_t x, int32_t y, int32_t z, const float r[8],

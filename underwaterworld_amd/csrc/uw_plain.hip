@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(kThreads) k_classify(float iso, const uw_table
 
 // Exclusive scan of per-chunk sizes into output offsets. One workgroup of 1024 threads.
 __global__ void __launch_bounds__(1024) k_offsets(uw_chunk_info* __restrict__ info, int32_t nchunks, uint64_t vcap,
-                                                  uint64_t icap, uw_totals* __restrict__ totals) {
+                                                  uint64_t icap, uint32_t ind16, uw_totals* __restrict__ totals) {
     __shared__ unsigned long long wv[16], wi[16];
     __shared__ unsigned int wn[16];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -197,6 +197,7 @@ __global__ void __launch_bounds__(1024) k_offsets(uw_chunk_info* __restrict__ in
                 r.flags |= UW_CHUNK_OVERFLOW;
                 ovf = 1;
             }
+            if (ind16 && v > 65535u) r.flags |= UW_CHUNK_INDEX_WRAP;
             info[c] = r;
         }
         carry_v += tv;
@@ -222,7 +223,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                                                    const uint32_t* __restrict__ voff, const uint32_t* __restrict__ act,
                                                    const uint32_t* __restrict__ ioff,
                                                    const uw_chunk_info* __restrict__ info, float* __restrict__ verts,
-                                                   uint32_t* __restrict__ inds, uint32_t vstride) {
+                                                   uint32_t* __restrict__ inds, uw_emit_opts eo) {
     const int32_t chunk = blockIdx.x;
     const uw_chunk_info ci = info[chunk];
     if (ci.flags & (UW_CHUNK_BLANK | UW_CHUNK_OVERFLOW)) return;
@@ -248,22 +249,25 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
         const uint32_t myv = voff[cbase + cell];
         const uint64_t vb = ci.vert_off + myv;
         const uint64_t ib = ci.ind_off + ioff[cbase + a];
+        const bool coll = eo.tri_normals || eo.tri_cells;
+        float q0[3] = {0.f, 0.f, 0.f}, q1[3] = {0.f, 0.f, 0.f};
         uint32_t seen = 0;
         for (int j = 0; j < nidx; ++j) {
             const uint32_t e = uint32_t(row >> (4 + 4 * j)) & 15u;
             const uint32_t bit = 1u << e;
+            const bool fresh = (owned & bit) && !(seen & bit);
+            float p[3];
+            if (fresh || coll) uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
             uint32_t id;
             if (owned & bit) {
                 const uint32_t rank = __popc(tab->before[cs][e] & owned);
                 id = myv + rank;
-                if (!(seen & bit)) {
-                    float p[3];
-                    uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
-                    float* dst = verts + vstride * (vb + rank);
+                if (fresh) {
+                    float* dst = verts + eo.vstride * (vb + rank);
                     dst[0] = p[0];
                     dst[1] = p[1];
                     dst[2] = p[2];
-                    if (vstride == 6) uw_vertex_color(p[2], uw_edge_b(e), float(extent), dst + 3);
+                    if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(e), float(extent), dst + 3);
                 }
             } else {
                 int32_t qx, qy, qz;
@@ -274,7 +278,27 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                 id = voff[cbase + q] + __popc(tab->before[qc][qe] & uw_owned_mask(qx, qy, qz));
             }
             seen |= bit;
-            inds[ib + j] = id;
+            uw_store_index(inds, eo.ind16, ib + j, id);
+            if (coll) {
+                const int k = j % 3;
+                if (k == 0) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) q0[d] = p[d];
+                } else if (k == 1) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) q1[d] = p[d];
+                } else {
+                    const uint64_t t = (ib + j) / 3;
+                    if (eo.tri_normals) {
+                        float nm[3];
+                        uw_tri_normal(q0, q1, p, nm);
+                        eo.tri_normals[3 * t] = nm[0];
+                        eo.tri_normals[3 * t + 1] = nm[1];
+                        eo.tri_normals[3 * t + 2] = nm[2];
+                    }
+                    if (eo.tri_cells) eo.tri_cells[t] = uint32_t(cell);
+                }
+            }
         }
     }
 }
@@ -372,7 +396,8 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
     if (!ctx) return UW_ERR_INVALID_ARG;
     if (int rc = uw_check_resolution(ctx, n)) return rc;
     if (nchunks < 0 || !out || !out->info || !out->totals || (nchunks > 0 && !coords) ||
-        (out->vert_cap > 0 && !out->verts) || (out->ind_cap > 0 && !out->inds) || out->vertex_format > 1)
+        (out->vert_cap > 0 && !out->verts) || (out->ind_cap > 0 && !out->inds) || out->vertex_format > 1 ||
+        out->index_format > 1)
         return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
     UW_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -390,20 +415,22 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
     hipLaunchKernelGGL(k_classify, dim3(uint32_t(nchunks)), dim3(kThreads), 0, st, f.iso, ctx->d_tables, n, s.below,
                        s.dens, cs, s.voff, s.act, s.ioff, out->info);
     hipLaunchKernelGGL(k_offsets, dim3(1), dim3(1024), 0, st, out->info, nchunks, out->vert_cap, out->ind_cap,
-                       out->totals);
+                       out->index_format == UW_INDEX_U16 ? 1u : 0u, out->totals);
     const float step = float(f.extent) / float(n);
     hipLaunchKernelGGL(k_emit, dim3(uint32_t(nchunks)), dim3(kThreads), 0, st, f.iso, f.extent, ctx->d_tables, coords,
                        n, step, s.dens, cs, s.voff, s.act, s.ioff, out->info, out->verts, out->inds,
-                       out->vertex_format == UW_VERTEX_XYZ_RGB ? 6u : 3u);
+                       uw_make_emit_opts(out));
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
 }
 
-int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, float* verts, uint64_t vert_cap,
-                        uint32_t* inds, uint64_t ind_cap, uw_chunk_info* info) {
-    if (!ctx || !info) return UW_ERR_INVALID_ARG;
+int uw_chunk_build_host_out(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, const uw_host_out* ho,
+                            uw_chunk_info* info) {
+    if (!ctx || !info || !ho) return UW_ERR_INVALID_ARG;
     if (int rc = uw_check_resolution(ctx, n)) return rc;
-    if ((vert_cap > 0 && !verts) || (ind_cap > 0 && !inds)) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    if ((ho->vert_cap > 0 && !ho->verts) || (ho->ind_cap > 0 && !ho->inds) || ho->vertex_format > 1 ||
+        ho->index_format > 1)
+        return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
     UW_HIP(ctx, hipSetDevice(ctx->device));
     const uint64_t vmax = uw_max_verts(n), imax = uw_max_inds(n);
     if (!ctx->d_coord1) {
@@ -414,27 +441,48 @@ int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t
     if (ctx->verts1_cap < vmax || ctx->inds1_cap < imax) {
         (void)hipFree(ctx->d_verts1);
         (void)hipFree(ctx->d_inds1);
+        (void)hipFree(ctx->d_norm1);
+        (void)hipFree(ctx->d_cells1);
         ctx->d_verts1 = nullptr;
         ctx->d_inds1 = nullptr;
+        ctx->d_norm1 = nullptr;
+        ctx->d_cells1 = nullptr;
         ctx->verts1_cap = ctx->inds1_cap = 0;
-        UW_HIP(ctx, hipMalloc(&ctx->d_verts1, vmax * 3 * sizeof(float)));
+        UW_HIP(ctx, hipMalloc(&ctx->d_verts1, vmax * 6 * sizeof(float)));
         UW_HIP(ctx, hipMalloc(&ctx->d_inds1, imax * sizeof(uint32_t)));
         ctx->verts1_cap = vmax;
         ctx->inds1_cap = imax;
     }
+    const bool want_norm = ho->tri_normals != nullptr, want_cells = ho->tri_cells != nullptr;
+    if (want_norm && !ctx->d_norm1) UW_HIP(ctx, hipMalloc(&ctx->d_norm1, ctx->inds1_cap * sizeof(float)));
+    if (want_cells && !ctx->d_cells1) UW_HIP(ctx, hipMalloc(&ctx->d_cells1, ctx->inds1_cap / 3 * sizeof(uint32_t)));
     const int32_t c[3] = {cx, cy, cz};
     UW_HIP(ctx, hipMemcpy(ctx->d_coord1, c, sizeof(c), hipMemcpyHostToDevice));
     uw_mesh_out out = {ctx->d_verts1, ctx->d_inds1, ctx->d_info1, ctx->d_totals1, ctx->verts1_cap, ctx->inds1_cap,
-                       UW_VERTEX_XYZ, 0u};
+                       ho->vertex_format, ho->index_format, want_norm ? ctx->d_norm1 : nullptr,
+                       want_cells ? ctx->d_cells1 : nullptr};
     if (int rc = uw_build_chunks_plain(ctx, ctx->d_coord1, 1, n, &out, nullptr, nullptr)) return rc;
     UW_HIP(ctx, hipMemcpy(info, ctx->d_info1, sizeof(uw_chunk_info), hipMemcpyDeviceToHost));
-    if (info->n_verts > vert_cap || info->n_inds > ind_cap)
+    if (info->n_verts > ho->vert_cap || info->n_inds > ho->ind_cap)
         return uw_fail(ctx, UW_ERR_OVERFLOW, "output capacity too small for this chunk");
+    const size_t vfl = ho->vertex_format == UW_VERTEX_XYZ_RGB ? 6 : 3;
+    const size_t isz = ho->index_format == UW_INDEX_U16 ? sizeof(uint16_t) : sizeof(uint32_t);
+    const size_t ntri = info->n_inds / 3;
     if (info->n_verts)
-        UW_HIP(ctx, hipMemcpy(verts, ctx->d_verts1, size_t(info->n_verts) * 3 * sizeof(float), hipMemcpyDeviceToHost));
-    if (info->n_inds)
-        UW_HIP(ctx, hipMemcpy(inds, ctx->d_inds1, size_t(info->n_inds) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        UW_HIP(ctx, hipMemcpy(ho->verts, ctx->d_verts1, size_t(info->n_verts) * vfl * sizeof(float),
+                              hipMemcpyDeviceToHost));
+    if (info->n_inds) UW_HIP(ctx, hipMemcpy(ho->inds, ctx->d_inds1, size_t(info->n_inds) * isz, hipMemcpyDeviceToHost));
+    if (ntri && want_norm)
+        UW_HIP(ctx, hipMemcpy(ho->tri_normals, ctx->d_norm1, ntri * 3 * sizeof(float), hipMemcpyDeviceToHost));
+    if (ntri && want_cells)
+        UW_HIP(ctx, hipMemcpy(ho->tri_cells, ctx->d_cells1, ntri * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return UW_OK;
+}
+
+int uw_chunk_build_host(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, float* verts, uint64_t vert_cap,
+                        uint32_t* inds, uint64_t ind_cap, uw_chunk_info* info) {
+    const uw_host_out ho = {verts, vert_cap, inds, ind_cap, UW_VERTEX_XYZ, UW_INDEX_U32, nullptr, nullptr};
+    return uw_chunk_build_host_out(ctx, cx, cy, cz, n, &ho, info);
 }
 
 }  // extern "C"

@@ -104,15 +104,48 @@ def permutation_table(seed: int) -> np.ndarray:
     return out
 
 
-class Chunk:
-    """One chunk, built through the engine in a single call (chunk.rs:80-349)."""
+def tri_ids_around(tri_cells: np.ndarray, n: int, local_pos_percent, range_: int) -> np.ndarray:
+    """uw_tris_around over one chunk's tri_cells: the chunk-local ids of the triangles that
+    Chunk::tris_around (chunk.rs:315-342) returns, in its order."""
+    tc = np.ascontiguousarray(tri_cells, np.uint32)
+    px, py, pz = (float(v) for v in local_pos_percent)
+    L = _lib.lib()
+    count = ctypes.c_uint32(0)
+    cap = len(tc)
+    ids = np.empty(max(cap, 1), np.uint32)
+    rc = L.uw_tris_around(tc.ctypes.data, len(tc), int(n), px, py, pz, int(range_), ids.ctypes.data, cap,
+                          ctypes.byref(count))
+    _lib.check(rc)
+    return ids[: count.value].copy()
 
-    def __init__(self, pos, resolution: int | None = None):
+
+class Tri:
+    """Collision triangle (util::Tri, util.rs:7-20): verts (3, 3) f32 world positions, normal (3,)."""
+    __slots__ = ("verts", "normal")
+
+    def __init__(self, verts, normal):
+        self.verts = verts
+        self.normal = normal
+
+
+class Chunk:
+    """One chunk, built through the engine in a single call (chunk.rs:80-349).
+
+    colors=True gives the reference's VertColor vertices (6 f32: position, sRGB colour); u16=True
+    its u16 index buffer (chunk.rs:243). Collision triangles (Build.tris) are always built, for
+    tris_around."""
+
+    def __init__(self, pos, resolution: int | None = None, colors: bool = False, u16: bool = False):
         self.pos = tuple(int(v) for v in pos)
         self.resolution = resolution
+        self.colors = colors
+        self.u16 = u16
         self._verts = None
         self._inds = None
         self._info = None
+        self._normals = None
+        self._tri_cells = None
+        self._n = None
         self._done = False
 
     @classmethod
@@ -131,19 +164,41 @@ class Chunk:
         n = int(self.resolution or perlin.params.get("resolution", 12))
         L = _lib.lib()
         vcap, icap = int(L.uw_max_verts(n)), int(L.uw_max_inds(n))
-        verts = np.empty((vcap, 3), np.float32)
-        inds = np.empty(icap, np.uint32)
+        verts = np.empty((vcap, 6 if self.colors else 3), np.float32)
+        inds = np.empty(icap, np.uint16 if self.u16 else np.uint32)
+        normals = np.empty((icap // 3, 3), np.float32)
+        cells = np.empty(icap // 3, np.uint32)
+        ho = _lib.UwHostOut(verts.ctypes.data, vcap, inds.ctypes.data, icap,
+                            VERTEX_XYZ_RGB if self.colors else VERTEX_XYZ, INDEX_U16 if self.u16 else INDEX_U32,
+                            normals.ctypes.data, cells.ctypes.data)
         info = _lib.UwChunkInfo()
-        rc = L.uw_chunk_build_host(perlin.ctx, self.pos[0], self.pos[1], self.pos[2], n,
-                                   verts.ctypes.data_as(ctypes.c_void_p), vcap,
-                                   inds.ctypes.data_as(ctypes.c_void_p), icap, ctypes.byref(info))
+        rc = L.uw_chunk_build_host_out(perlin.ctx, self.pos[0], self.pos[1], self.pos[2], n, ctypes.byref(ho),
+                                       ctypes.byref(info))
         _lib.check(rc, perlin.ctx)
         self._info = info
+        self._n = n
         if info.n_inds > 0:
+            nt = info.n_inds // 3
             self._verts = verts[: info.n_verts].copy()
             self._inds = inds[: info.n_inds].copy()
+            self._normals = normals[:nt].copy()
+            self._tri_cells = cells[:nt].copy()
         self._done = True
         return True
+
+    def tris_around(self, local_pos_percent, range_: int) -> list:
+        """Chunk::tris_around (chunk.rs:315-342): collision triangles of the cells within range_
+        cells of local_pos_percent (chunk-local position / chunk extent), in the reference's order."""
+        if self._tri_cells is None:
+            return []
+        ids = tri_ids_around(self._tri_cells, self._n, local_pos_percent, range_)
+        pos = self._verts[:, :3]
+        tri = pos[self._inds.astype(np.int64)].reshape(-1, 3, 3)
+        return [Tri(tri[t], self._normals[t]) for t in ids]
+
+    def collision(self):
+        """(tri_cells (T,) u32, normals (T, 3) f32) of the chunk's collision triangles."""
+        return self._tri_cells, self._normals
 
     def not_blank(self) -> bool:
         return self._verts is not None
@@ -176,6 +231,8 @@ INFO_DTYPE = np.dtype([("flags", "<u4"), ("n_verts", "<u4"), ("n_inds", "<u4"), 
                        ("vert_off", "<u8"), ("ind_off", "<u8")])
 VERTEX_XYZ = 0      # 3 floats per vertex: world position
 VERTEX_XYZ_RGB = 1  # 6 floats: position + sRGB colour, the reference's VertColor (draw.rs:4-9)
+INDEX_U32 = 0
+INDEX_U16 = 1       # the reference's index buffer type (chunk.rs:243)
 TOTALS_DTYPE = np.dtype([("n_verts", "<u8"), ("n_inds", "<u8"), ("n_nonblank", "<u4"), ("overflow", "<u4")])
 
 
@@ -187,6 +244,8 @@ class MeshBatch:
     info_raw: "torch.Tensor"  # (nchunks * 32,) u8
     totals_raw: "torch.Tensor"  # (24,) u8
     cases: "torch.Tensor | None"
+    tri_normals: "torch.Tensor | None" = None  # (ind_cap / 3, 3) f32, triangle ind_off / 3 + t
+    tri_cells: "torch.Tensor | None" = None    # (ind_cap / 3,) i32 (u32 bits), chunk-local cell
 
     def info(self) -> np.ndarray:
         return self.info_raw.cpu().numpy().view(INFO_DTYPE)
@@ -199,23 +258,42 @@ class MeshBatch:
         r = (info if info is not None else self.info())[i]
         v0, nv, i0, ni = int(r["vert_off"]), int(r["n_verts"]), int(r["ind_off"]), int(r["n_inds"])
         v = self.verts[v0:v0 + nv].cpu().numpy()
-        ix = self.inds[i0:i0 + ni].cpu().numpy().view(np.uint32)
+        ix = self.inds[i0:i0 + ni].cpu().numpy()
+        ix = ix.view(np.uint16 if ix.dtype == np.int16 else np.uint32)
         return v, ix
 
+    def chunk_collision(self, i: int, info=None):
+        """(tri_cells (T,) u32, normals (T, 3) f32) of chunk i (built with collision=True)."""
+        r = (info if info is not None else self.info())[i]
+        t0, nt = int(r["ind_off"]) // 3, int(r["n_inds"]) // 3
+        cells = self.tri_cells[t0:t0 + nt].cpu().numpy().view(np.uint32)
+        return cells, self.tri_normals[t0:t0 + nt].cpu().numpy()
 
-def _alloc_out(nchunks: int, vert_cap: int, ind_cap: int, device, colors: bool = False):
+
+def _alloc_out(nchunks: int, vert_cap: int, ind_cap: int, device, colors: bool = False, u16: bool = False):
     import torch
     verts = torch.empty((max(vert_cap, 1), 6 if colors else 3), dtype=torch.float32, device=device)
-    inds = torch.empty((max(ind_cap, 1),), dtype=torch.int32, device=device)
+    inds = torch.empty((max(ind_cap, 1),), dtype=torch.int16 if u16 else torch.int32, device=device)
     info = torch.empty((max(nchunks, 1) * 32,), dtype=torch.uint8, device=device)
     totals = torch.empty((24,), dtype=torch.uint8, device=device)
     return verts, inds, info, totals
 
 
-def _mesh_out(verts, inds, info, totals, vert_cap, ind_cap):
+def _alloc_collision(ind_cap: int, device):
+    import torch
+    nt = max(ind_cap // 3, 1)
+    return (torch.empty((nt, 3), dtype=torch.float32, device=device),
+            torch.empty((nt,), dtype=torch.int32, device=device))
+
+
+def _mesh_out(verts, inds, info, totals, vert_cap, ind_cap, collision=None):
+    import torch
     fmt = VERTEX_XYZ_RGB if verts.shape[1] == 6 else VERTEX_XYZ
+    ifmt = INDEX_U16 if inds.dtype == torch.int16 else INDEX_U32
+    nrm, cells = collision if collision is not None else (None, None)
     return _lib.UwMeshOut(verts.data_ptr(), inds.data_ptr(), info.data_ptr(), totals.data_ptr(), vert_cap, ind_cap,
-                          fmt, 0)
+                          fmt, ifmt, nrm.data_ptr() if nrm is not None else None,
+                          cells.data_ptr() if cells is not None else None)
 
 
 def _coords_tensor(coords, device):
@@ -228,7 +306,8 @@ def _coords_tensor(coords, device):
 
 
 def build_chunks_plain(perlin: Perlin, coords, n: int, vert_cap: int | None = None, ind_cap: int | None = None,
-                       want_cases: bool = False, stream=None, colors: bool = False) -> MeshBatch:
+                       want_cases: bool = False, stream=None, colors: bool = False, u16: bool = False,
+                       collision: bool = False) -> MeshBatch:
     import torch
     dev = torch.device("cuda", perlin.device)
     ct = _coords_tensor(coords, dev)
@@ -238,13 +317,14 @@ def build_chunks_plain(perlin: Perlin, coords, n: int, vert_cap: int | None = No
         vert_cap = int(L.uw_max_verts(n)) * nch
     if ind_cap is None:
         ind_cap = int(L.uw_max_inds(n)) * nch
-    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev, colors)
+    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev, colors, u16)
+    coll = _alloc_collision(ind_cap, dev) if collision else None
     cases = torch.empty((max(nch, 1) * n ** 3,), dtype=torch.uint8, device=dev) if want_cases else None
-    out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
+    out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap, coll)
     rc = L.uw_build_chunks_plain(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), nch, n, ctypes.byref(out),
                                  ctypes.c_void_p(cases.data_ptr() if cases is not None else 0), _stream_ptr(stream))
     _lib.check(rc, perlin.ctx)
-    return MeshBatch(verts, inds, info[: nch * 32], totals, cases)
+    return MeshBatch(verts, inds, info[: nch * 32], totals, cases, *(coll or (None, None)))
 
 
 def _res_tensor(res, nch, device):
@@ -263,7 +343,7 @@ def batch_capacity(res) -> tuple[int, int]:
 
 
 def build_chunks_batched(perlin: Perlin, coords, res, vert_cap: int | None = None, ind_cap: int | None = None,
-                         stream=None, colors: bool = False) -> MeshBatch:
+                         stream=None, colors: bool = False, u16: bool = False, collision: bool = False) -> MeshBatch:
     """One persistent launch over a mixed-resolution chunk set (res: cells per axis per chunk).
     colors=True writes the reference's VertColor layout (position + sRGB colour per vertex)."""
     import torch
@@ -275,12 +355,13 @@ def build_chunks_batched(perlin: Perlin, coords, res, vert_cap: int | None = Non
         vc, ic = batch_capacity(rt[:nch].cpu().numpy() if nch else [])
         vert_cap = vc if vert_cap is None else vert_cap
         ind_cap = ic if ind_cap is None else ind_cap
-    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev, colors)
-    out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap)
+    verts, inds, info, totals = _alloc_out(nch, vert_cap, ind_cap, dev, colors, u16)
+    coll = _alloc_collision(ind_cap, dev) if collision else None
+    out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap, coll)
     rc = _lib.lib().uw_build_chunks_batched(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), ctypes.c_void_p(rt.data_ptr()),
                                             nch, ctypes.byref(out), _stream_ptr(stream))
     _lib.check(rc, perlin.ctx)
-    return MeshBatch(verts, inds, info[: nch * 32], totals, None)
+    return MeshBatch(verts, inds, info[: nch * 32], totals, None, *(coll or (None, None)))
 
 
 class BatchedGraph:
