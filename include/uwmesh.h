@@ -279,6 +279,87 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
 int uw_graph_launch(uw_graph* g, void* stream);
 void uw_graph_destroy(uw_graph* g);
 
+/* ---------------------------------------------------------------------------------------------
+ * World streaming (world.rs:64-241) with a batched generation step: World::update re-plans the
+ * neighbourhood like the reference, but instead of building one chunk per frame
+ * (build_full_step / build_step, world.rs:113-145) it builds up to max_batch pending candidates,
+ * in the reference's priority (pop) order, in ONE batched launch. Chunks, the render list and the
+ * eviction walk keep the reference's rules; results enter them in pop order.
+ *
+ * Matrices are cgmath Matrix4<f32> as 16 floats, column-major (m[4 * col + row]), i.e. what
+ * Camera::chunk_generation_frustum_matrix(55 deg / 90 deg) returns (camera.rs:58-62).
+ * ------------------------------------------------------------------------------------------- */
+typedef struct uw_world_config {
+    int32_t view_dist;       /* world.rs:8   VIEW_DIST       4  */
+    int32_t generation_dist; /* world.rs:9   GENERATION_DIST 5  */
+    int32_t keep_dist;       /* world.rs:10  KEEP_DIST       6  */
+    int32_t max_z, min_z;    /* world.rs:11-12               2, -2 */
+    float recheck_dist;      /* world.rs:5   4.0  */
+    float recheck_angle;     /* world.rs:6   0.33 rad */
+    int32_t resolution;      /* cells per chunk axis (chunk.rs:6: 12) */
+    int32_t max_batch;       /* chunks built per update; 0 = every pending candidate */
+    uint32_t vertex_format;  /* enum uw_vertex_format of the stored meshes */
+    uint32_t index_format;   /* enum uw_index_format */
+    int32_t collision;       /* nonzero: also store collision triangles (for tris_around) */
+} uw_world_config;
+
+/* One planned candidate: chunk position and its GenPrio (world.rs:29-34). */
+typedef struct uw_world_candidate {
+    int32_t pos[3];
+    float dist;
+    float z;
+    int32_t in_view;
+    int32_t in_gen;
+} uw_world_candidate;
+
+/* The reference's constants; max_batch 0, xyz vertices, u32 indices, collision on. */
+void uw_world_config_reference(uw_world_config* cfg);
+
+/* World::update_nearby (world.rs:149-228), host-only. known: nknown chunk positions (int32[3])
+ * with their not_blank flags (uint8, may be NULL when nknown == 0). Writes the render list
+ * (int32[3] each, at most render_cap) and the generation candidates in pop order (most important
+ * first, at most cand_cap); *n_render / *n_cand receive the full counts (UW_ERR_OVERFLOW if a
+ * capacity was too small). Priority ties, which the reference orders arbitrarily
+ * (sort_unstable_by), come out in reverse generation order. */
+int uw_world_plan(const uw_world_config* cfg, const float sub_pos[3], const float view_proj[16],
+                  const float gen_proj[16], const int32_t* known, const uint8_t* known_not_blank, int32_t nknown,
+                  int32_t* render, int32_t render_cap, int32_t* n_render, uw_world_candidate* cands,
+                  int32_t cand_cap, int32_t* n_cand);
+
+typedef struct uw_world uw_world;
+int uw_world_create(uw_ctx* ctx, const uw_world_config* cfg, uw_world** out);
+void uw_world_destroy(uw_world* w);
+
+/* World::update (world.rs:93-111): remove_far_way, re-plan if reset / moved / turned, then one
+ * batched build of up to max_batch candidates. Synchronous: on return the new chunks are stored
+ * and the render list is final. *built (may be NULL) receives the number of chunks built. */
+int uw_world_update(uw_world* w, const float sub_pos[3], const float sub_bearing[3], const float view_proj[16],
+                    const float gen_proj[16], int sub_reset, int32_t* built);
+
+/* world.rs:243-247 */
+int32_t uw_world_generate_count(const uw_world* w);
+int32_t uw_world_render_count(const uw_world* w);
+int32_t uw_world_total_count(const uw_world* w);
+/* chunks_to_render (world.rs:243): up to cap positions (int32[3] each); returns the count. */
+int32_t uw_world_chunks_to_render(const uw_world* w, int32_t* out, int32_t cap);
+
+/* World::get_chunk (world.rs:89-91). Returns UW_OK and the chunk's record, or UW_ERR_INVALID_ARG
+ * when the chunk is not stored. Device pointers (valid until the chunk is evicted) to its
+ * vertices, indices and, when the world stores them, collision records; NULL for a blank chunk. */
+typedef struct uw_world_chunk {
+    uw_chunk_info info;   /* vert_off / ind_off are relative to this chunk's own buffers (0) */
+    const float* verts;
+    const void* inds;
+    const float* tri_normals;
+    const uint32_t* tri_cells;
+} uw_world_chunk;
+int uw_world_get_chunk(const uw_world* w, int32_t x, int32_t y, int32_t z, uw_world_chunk* out);
+
+/* Copies a stored chunk's buffers to host memory (the upload source of chunk.rs:291-303);
+ * buffers may be NULL to skip them. UW_ERR_OVERFLOW if a capacity is too small. */
+int uw_world_read_chunk(const uw_world* w, int32_t x, int32_t y, int32_t z, const uw_host_out* out,
+                        uw_chunk_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,9 +21,16 @@ def test_point_queries_match_iso_at(preset):
     got = density_at(Perlin(p), pts)
     torch.cuda.synchronize()
     want = O.iso_at(p, pts)
-    err = float(np.max(np.abs(got.cpu().numpy().astype(np.float64) - want)))
-    print(f"\n[{preset}] max |rho_gpu - iso_at| = {err:.3g}")
-    assert err <= 1e-6
+    diff = np.abs(got.cpu().numpy().astype(np.float64) - want)
+    # 1e-6 absolute, or one f32 ulp of the result where |rho| is large: far above or below the
+    # world's z range (|z| > 2 noise units) the z bias alone makes |rho| reach 20, where one ulp
+    # is 1.9e-6.
+    tol = np.maximum(1e-6, np.spacing(np.abs(want).astype(np.float32)).astype(np.float64))
+    inside = np.abs(pts[:, 2]) <= 2.0
+    print(f"\n[{preset}] max |rho_gpu - iso_at| = {diff.max():.3g} (|z| <= 2: {diff[inside].max():.3g}), "
+          f"max diff/tol = {(diff / tol).max():.3g}")
+    assert (diff <= tol).all()
+    assert diff[inside].max() <= 1e-6
 
 
 def test_point_queries_agree_with_grid_samples():

@@ -78,6 +78,23 @@ class UwHostOut(ctypes.Structure):
                 ("tri_cells", c_void_p)]
 
 
+class UwWorldConfig(ctypes.Structure):
+    _fields_ = [("view_dist", ctypes.c_int32), ("generation_dist", ctypes.c_int32), ("keep_dist", ctypes.c_int32),
+                ("max_z", ctypes.c_int32), ("min_z", ctypes.c_int32), ("recheck_dist", ctypes.c_float),
+                ("recheck_angle", ctypes.c_float), ("resolution", ctypes.c_int32), ("max_batch", ctypes.c_int32),
+                ("vertex_format", ctypes.c_uint32), ("index_format", ctypes.c_uint32), ("collision", ctypes.c_int32)]
+
+
+class UwWorldCandidate(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_int32 * 3), ("dist", ctypes.c_float), ("z", ctypes.c_float),
+                ("in_view", ctypes.c_int32), ("in_gen", ctypes.c_int32)]
+
+
+class UwWorldChunk(ctypes.Structure):
+    _fields_ = [("info", UwChunkInfo), ("verts", c_void_p), ("inds", c_void_p), ("tri_normals", c_void_p),
+                ("tri_cells", c_void_p)]
+
+
 # name -> (restype, argtypes); every symbol include/uwmesh.h declares.
 SIGNATURES = {
     "uw_params_reference": (None, [P(UwParams), ctypes.c_uint32]),
@@ -114,6 +131,22 @@ SIGNATURES = {
     "uw_adjoint_backward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p,
                                            c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p, c_void_p]),
     "uw_graph_destroy": (None, [c_void_p]),
+    "uw_world_config_reference": (None, [P(UwWorldConfig)]),
+    "uw_world_plan": (ctypes.c_int, [P(UwWorldConfig), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                     ctypes.c_int32, c_void_p, ctypes.c_int32, P(ctypes.c_int32), P(UwWorldCandidate),
+                                     ctypes.c_int32, P(ctypes.c_int32)]),
+    "uw_world_create": (ctypes.c_int, [c_void_p, P(UwWorldConfig), P(c_void_p)]),
+    "uw_world_destroy": (None, [c_void_p]),
+    "uw_world_update": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_int,
+                                       P(ctypes.c_int32)]),
+    "uw_world_generate_count": (ctypes.c_int32, [c_void_p]),
+    "uw_world_render_count": (ctypes.c_int32, [c_void_p]),
+    "uw_world_total_count": (ctypes.c_int32, [c_void_p]),
+    "uw_world_chunks_to_render": (ctypes.c_int32, [c_void_p, c_void_p, ctypes.c_int32]),
+    "uw_world_get_chunk": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          P(UwWorldChunk)]),
+    "uw_world_read_chunk": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, P(UwHostOut),
+                                           P(UwChunkInfo)]),
 }
 
 _lib = None
