@@ -159,6 +159,11 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("underwaterworld_amd/libuwmesh.so is not built: run "
                                "`python -m underwaterworld_amd._build` (or __graft_entry__.build())")
+        # torch bundles its own HIP runtime under the same soname (libamdhip64.so.7). Loading torch
+        # first makes this library bind to that runtime, so the process has one HIP runtime and
+        # torch's streams, allocations and synchronisation cover this library's launches. (Loaded
+        # the other way round, a second runtime comes up and torch finds no GPU.)
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

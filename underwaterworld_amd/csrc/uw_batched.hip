@@ -234,7 +234,7 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
     }
 }
 
-template <int BLOCK, int OMAX>
+template <int BLOCK, int OCT>
 __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     constexpr int W = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         if (task >= uint32_t(a.nchunks)) break;
         const int32_t c = int32_t(a.order[task]);
         const int32_t n = a.res[c];
-        if (n < 1 || n > a.max_res || O > OMAX) {
+        if (n < 1 || n > a.max_res || !uw_octaves<OCT>::accepts(O)) {
             if (tid == 0) {
                 uw_chunk_info r = {UW_CHUNK_INVALID | UW_CHUNK_BLANK, 0, 0, 0, 0, 0};
                 a.info[c] = r;
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
 
         if (nslab == 1 && C <= a.list_cap) {
             // ---------- one density evaluation, one classification pass ----------
-            const int bits = uw_noise_planes<OMAX>(a.f, T, P, GP, L, 0, L, DENS, ns, tid, BLOCK);
+            const int bits = uw_noise_planes<OCT>(a.f, T, P, GP, L, 0, L, DENS, ns, tid, BLOCK);
             if (bits) atomicOr(&s_bits, uint32_t(bits));
             __syncthreads();
             const uint32_t sb = s_bits;
@@ -429,7 +429,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         unsigned long long cnt = 0;  // nv | nt << 24 | act << 48 (per thread)
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
-            below |= uw_noise_planes<OMAX>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
+            below |= uw_noise_planes<OCT>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
                      UW_BELOW_NOT_ABOVE;
             __syncthreads();
             for (int32_t g = xa * nn + tid; g < xe * nn; g += BLOCK) {
@@ -480,7 +480,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
             if (nslab > 1) {
-                uw_noise_planes<OMAX>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+                uw_noise_planes<OCT>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
                 __syncthreads();
             }
             uint32_t list_n = 0;
@@ -535,22 +535,24 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
 using BatchedKernel = void (*)(BatchArgs);
 
 template <int BLOCK>
-BatchedKernel pick_octaves(int omax) {
-    if (omax <= 4) return k_batched<BLOCK, 4>;
-    if (omax <= 8) return k_batched<BLOCK, 8>;
-    return k_batched<BLOCK, 16>;
-}
-
-BatchedKernel pick_kernel(int block, int omax) {
-    switch (block) {
-        case 128: return pick_octaves<128>(omax);
-        case 192: return pick_octaves<192>(omax);
-        case 512: return pick_octaves<512>(omax);
-        default: return pick_octaves<256>(omax);
+BatchedKernel pick_octaves(int octaves) {
+    switch (octaves) {
+        case 3: return k_batched<BLOCK, 3>;
+        case 4: return k_batched<BLOCK, 4>;
+        case 6: return k_batched<BLOCK, 6>;
+        case 8: return k_batched<BLOCK, 8>;
+        default: return k_batched<BLOCK, 0>;
     }
 }
 
-int omax_of(int octaves) { return octaves <= 4 ? 4 : (octaves <= 8 ? 8 : 16); }
+BatchedKernel pick_kernel(int block, int octaves) {
+    switch (block) {
+        case 128: return pick_octaves<128>(octaves);
+        case 192: return pick_octaves<192>(octaves);
+        case 512: return pick_octaves<512>(octaves);
+        default: return pick_octaves<256>(octaves);
+    }
+}
 
 // Compacted-list capacity: every chunk built in one pass (one slab, n^3 <= kRing cells) must fit,
 // and a round of `block` cells must fit on top of a flushable list.
@@ -617,7 +619,7 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         ctx->batch_order_cap = size_t(nchunks);
     }
     if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, sizeof(uint32_t)));
-    bl->k = pick_kernel(tune.block, omax_of(octaves));
+    bl->k = pick_kernel(tune.block, octaves);
     bl->lds = lds_fixed_bytes(size_t(list_cap_for(tune))) + lds_table_bytes(octaves) + dens_bytes_for(tune);
     UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(bl->k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(bl->lds)));

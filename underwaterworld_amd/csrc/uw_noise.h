@@ -184,16 +184,28 @@ __device__ __forceinline__ void uw_build_chunk_tables(const uw_field& f, const u
 #define UW_BELOW_LT 2
 #define UW_BELOW_GE 4
 
+// Octave-count specialisation of the density kernels: OCT > 0 compiles for exactly OCT octaves
+// (the common counts, so the octave loop unrolls with no per-octave predicates); OCT == 0 reads
+// the count from the field (at most UW_MAX_OCTAVES).
+template <int OCT>
+struct uw_octaves {
+    static constexpr int kMax = OCT > 0 ? OCT : UW_MAX_OCTAVES;
+    __device__ __forceinline__ static int32_t count(const uw_field& f) { return OCT > 0 ? OCT : f.octaves; }
+    __host__ __device__ static constexpr bool accepts(int32_t o) { return OCT > 0 ? o == OCT : o <= UW_MAX_OCTAVES; }
+};
+
 // Densities of x-planes [xa, xb) of a chunk whose tables are in T, one thread per (x, y) line.
 // Plane x is written to dst + slot(x) * L * L with slot(x) = x % nslot, z fastest.
 // Returns, per thread, the OR of the UW_BELOW_* bits of its samples.
-template <int OMAX>
+template <int OCT>
 __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uint8_t* P,
                                                const float4* GP, int32_t L, int32_t xa, int32_t xb, float* dst,
                                                int32_t nslot, int tid, int nthr) {
-    const int32_t O = f.octaves;
+    constexpr int OMAX = uw_octaves<OCT>::kMax;
+    const int32_t O = uw_octaves<OCT>::count(f);
     const int32_t lines = (xb - xa) * L;
-    int below = 0;
+    // running extremes of the thread's samples; the sign bits follow from them at the end
+    float vmin = INFINITY, vmax = -INFINITY;
     for (int32_t t = tid; t < lines; t += nthr) {
         const int32_t ix = xa + t / L, iy = t - (t / L) * L;
         float* out = dst + (ix % nslot) * L * L + iy * L;
@@ -210,24 +222,41 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
                 zcur[k] = 0xFFFFFFFFu;
             }
         }
+        // The z entries are wave-uniform: those of sample iz + 1 are fetched while sample iz is
+        // evaluated (the last fetch repeats index L - 1 and is discarded).
+        uw_axis Zn[OMAX];
+        float adjn = T.ADJ[0], postn = T.POST[0];
+#pragma unroll
+        for (int k = 0; k < OMAX; ++k)
+            if (k < O) Zn[k] = T.AZ[k * L];
         for (int32_t iz = 0; iz < L; ++iz) {
+            uw_axis Z[OMAX];
+#pragma unroll
+            for (int k = 0; k < OMAX; ++k) Z[k] = Zn[k];
+            const float adj = adjn, post = postn;
+            const int32_t izn = iz + 1 < L ? iz + 1 : iz;
+#pragma unroll
+            for (int k = 0; k < OMAX; ++k)
+                if (k < O) Zn[k] = T.AZ[k * L + izn];
+            adjn = T.ADJ[izn];
+            postn = T.POST[izn];
             float total = 0.0f;
 #pragma unroll
             for (int k = 0; k < OMAX; ++k) {
                 if (k < O) {
-                    const uw_axis Z = T.AZ[k * L + iz];
-                    if (Z.bits != zcur[k]) {
-                        st[k] = uw_line_setup(T.AX[k * L + ix], T.AY[k * L + iy], bb[k], Z.bits, GP);
-                        zcur[k] = Z.bits;
+                    if (Z[k].bits != zcur[k]) {
+                        st[k] = uw_line_setup(T.AX[k * L + ix], T.AY[k * L + iy], bb[k], Z[k].bits, GP);
+                        zcur[k] = Z[k].bits;
                     }
-                    total = fmaf(f.weight[k], uw_line_sample(st[k], Z, f.clamp_lo, f.clamp_hi), total);
+                    total = fmaf(f.weight[k], uw_line_sample(st[k], Z[k], f.clamp_lo, f.clamp_hi), total);
                 }
             }
-            const float rho = (T.ADJ[iz] + total) + T.POST[iz];
+            const float rho = (adj + total) + post;
             out[iz] = rho;
-            below |= (rho > f.iso ? 0 : UW_BELOW_NOT_ABOVE) | (rho < f.iso ? UW_BELOW_LT : 0) |
-                     (rho >= f.iso ? UW_BELOW_GE : 0);
+            vmin = fminf(vmin, rho);
+            vmax = fmaxf(vmax, rho);
         }
     }
-    return below;
+    return (vmin <= f.iso ? UW_BELOW_NOT_ABOVE : 0) | (vmin < f.iso ? UW_BELOW_LT : 0) |
+           (vmax >= f.iso ? UW_BELOW_GE : 0);
 }

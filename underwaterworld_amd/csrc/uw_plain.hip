@@ -29,7 +29,7 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 // Density grid. One workgroup per (chunk, slab of `planes` x-planes): the chunk's per-axis octave
 // tables are built in LDS, the slab's densities are evaluated line by line (uw_noise_planes) into
 // LDS and then stored to HBM with contiguous, coalesced writes.
-template <int OMAX>
+template <int OCT>
 __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_tables* __restrict__ tab,
                                                          const int32_t* __restrict__ coords, int32_t n,
                                                          float step, int32_t planes, int32_t slabs,
@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     uw_build_chunk_tables(f, P, cx, cy, cz, L, step, T, threadIdx.x, blockDim.x);
     __syncthreads();
     // plane x lands in slot x % planes = x - x0
-    const int bits = uw_noise_planes<OMAX>(f, T, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
+    const int bits = uw_noise_planes<OCT>(f, T, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
     if (bits) atomicOr(&s_bits, uint32_t(bits));
     __syncthreads();
     const int32_t LL = L * L;
@@ -320,9 +320,13 @@ using NoiseKernel = void (*)(uw_field, const uw_tables*, const int32_t*, int32_t
                              uint32_t*);
 
 NoiseKernel pick_noise(int octaves) {
-    if (octaves <= 4) return k_noise_grid<4>;
-    if (octaves <= 8) return k_noise_grid<8>;
-    return k_noise_grid<16>;
+    switch (octaves) {
+        case 3: return k_noise_grid<3>;
+        case 4: return k_noise_grid<4>;
+        case 6: return k_noise_grid<6>;
+        case 8: return k_noise_grid<8>;
+        default: return k_noise_grid<0>;
+    }
 }
 
 int ensure_plain_scratch(uw_ctx* ctx, int32_t nchunks, int32_t n) {
