@@ -21,12 +21,12 @@
 
 #include "uw_mesh.h"
 #include "uw_noise.h"
+#include "uw_wave.h"
 
 namespace {
 
 constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
 constexpr int kMaxL = kMaxN + 1;
-constexpr int kLPad = (kMaxL + 3) & ~3;  // ADJ/POST rows padded to keep later tables 16-byte aligned
 constexpr int kRing = 2048;       // cell ring (vertex base << 8 | case), power of two
 constexpr int kMinList = 512;     // smallest compacted active-cell list
 constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
@@ -44,6 +44,8 @@ struct BatchArgs {
     float* verts;
     uint32_t* inds;
     uw_vertex_tape* tape;  // optional per-vertex record for the adjoint
+    const unsigned char* ztab;  // per-chunk z tables (uw_ztab layout), written by k_ztables
+    int32_t zstride;            // samples per z-table row: max_res + 1
     uw_emit_opts eo;       // vertex / index formats and optional collision records
     uint64_t vcap, icap;
     int32_t max_res;      // largest resolution accepted by this launch (sizes the LDS)
@@ -92,12 +94,7 @@ __device__ __forceinline__ unsigned long long block_scan_u64(unsigned long long 
                                                              unsigned long long* lds) {
     constexpr int W = BLOCK / 64;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    unsigned long long inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const unsigned long long t = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += t;
-    }
+    const unsigned long long inc = uw_wave_incl_scan(v);
     if (lane == 63) lds[wave] = inc;
     __syncthreads();
     unsigned long long base = 0, sum = 0;
@@ -149,9 +146,9 @@ __device__ __forceinline__ CellCounts classify_cell(const float* ring, int32_t n
 // LDS carve-up (bytes): fixed part, then the per-octave axis tables, then the density ring.
 __host__ __device__ constexpr size_t lds_fixed_bytes(size_t list_cap) {
     return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + size_t(kRing) * 4 + 2 * list_cap * 4 +
-           2 * kLPad * 4 /*ADJ, POST*/ + 256 /*P*/;
+           256 /*P*/;
 }
-__host__ __device__ inline size_t lds_table_bytes(int octaves) { return 3 * size_t(octaves) * kMaxL * sizeof(uw_axis); }
+__host__ __device__ inline size_t lds_table_bytes(int octaves) { return 2 * size_t(octaves) * kMaxL * sizeof(uw_axis); }
 
 // Emission of one active cell: owned-edge lerps into verts (and the adjoint tape), then the
 // cell's indices. vbase: chunk-local vertex id of the cell's first owned record; ibase: global
@@ -161,7 +158,8 @@ template <class OwnerVoff>
 __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t L, int32_t x, int32_t y, int32_t z,
                                           uint32_t cs, const float r[8], uint32_t myv, uint64_t vo, uint64_t ib,
                                           float step, float ox, float oy, float oz, const uint64_t* TRI,
-                                          const uint32_t* RING, uint32_t ring_mask, OwnerVoff owner_voff) {
+                                          const uint32_t* RING, uint32_t ring_mask, OwnerVoff owner_voff,
+                                          uint64_t vlim, uint64_t ilim) {
     const float iso = a.f.iso;
     const uw_tables* tab = a.tab;
     const uint32_t owned = uw_owned_mask(x, y, z);
@@ -182,7 +180,8 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
         if (owned & bit) {
             const uint32_t rank = __popc(uint32_t(tab->before[cs][ed]) & owned);
             id = myv + rank;
-            if (fresh) {
+            // vlim / ilim end the chunk's reserved ranges: writes stay inside them whatever the ids
+            if (fresh && vb + rank < vlim) {
                 float* dst = a.verts + eo.vstride * (vb + rank);
                 dst[0] = p[0];
                 dst[1] = p[1];
@@ -210,6 +209,7 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
             id = owner_voff(q, qw) + __popc(uint32_t(tab->before[qw & 255u][qe]) & uw_owned_mask(qx, qy, qz));
         }
         seen |= bit;
+        if (ib + j >= ilim) continue;
         uw_store_index(a.inds, eo.ind16, ib + j, id);
         if (coll) {
             const int k = j % 3;
@@ -245,13 +245,10 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     uint32_t* RING = LUT + 256;
     uint32_t* LIST = RING + kRing;
     uint32_t* LIOFF = LIST + a.list_cap;
-    float* ADJ = reinterpret_cast<float*>(LIOFF + a.list_cap);
-    float* POST = ADJ + kLPad;
-    uint8_t* P = reinterpret_cast<uint8_t*>(POST + kLPad);
+    uint8_t* P = reinterpret_cast<uint8_t*>(LIOFF + a.list_cap);  // list_cap is a multiple of 4
     uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);
     uw_axis* AY = AX + O * kMaxL;
-    uw_axis* AZ = AY + O * kMaxL;
-    float* DENS = reinterpret_cast<float*>(AZ + O * kMaxL);
+    float* DENS = reinterpret_cast<float*>(AY + O * kMaxL);
     __shared__ unsigned long long scan_lds[W];
     __shared__ uint32_t wtot[W][3];
     __shared__ uint32_t s_task, s_bits;
@@ -292,8 +289,9 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         const int32_t cx = a.coords[3 * c], cy = a.coords[3 * c + 1], cz = a.coords[3 * c + 2];
         const float step = float(a.f.extent) / float(n);
         const float ox = float(a.f.extent * cx), oy = float(a.f.extent * cy), oz = float(a.f.extent * cz);
-        const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
-        uw_build_chunk_tables(a.f, P, cx, cy, cz, L, step, T, tid, BLOCK);
+        const uw_chunk_tables T = {AX, AY, nullptr, nullptr, nullptr};
+        uw_build_xy_tables(a.f, P, cx, cy, L, step, AX, AY, tid, BLOCK);
+        const uw_ztab Zt = uw_ztab_of(a.ztab, __builtin_amdgcn_readfirstlane(c), O, a.zstride);
         __syncthreads();
 
         const int32_t sx = a.sx[n];
@@ -302,7 +300,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
 
         if (nslab == 1 && C <= a.list_cap) {
             // ---------- one density evaluation, one classification pass ----------
-            const int bits = uw_noise_planes<OCT>(a.f, T, P, GP, L, 0, L, DENS, ns, tid, BLOCK);
+            const int bits = uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, 0, L, DENS, ns, tid, BLOCK);
             if (bits) atomicOr(&s_bits, uint32_t(bits));
             __syncthreads();
             const uint32_t sb = s_bits;
@@ -328,27 +326,23 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                     uw_cell_xyz(g, dn, dnn, &x, &y, &z);
                     cc = classify_cell(DENS, ns, L, x, y, z, iso, LUT);
                 }
-                const unsigned long long v = (unsigned long long)cc.nv | ((unsigned long long)cc.nt << 16) |
-                                             ((unsigned long long)cc.act << 32);
-                unsigned long long inc = v;
-#pragma unroll
-                for (int d = 1; d < 64; d <<= 1) {
-                    const unsigned long long t = __shfl_up(inc, d, 64);
-                    if (lane >= d) inc += t;
-                }
-                const unsigned long long ex = inc - v;
+                // per wave of 64 cells: vertices <= 768 (10 bits), triangles <= 320 (9 bits),
+                // active cells <= 64 (7 bits), so one 32-bit scan carries all three
+                const uint32_t v = cc.nv | (cc.nt << 10) | (cc.act << 19);
+                const uint32_t inc = uw_wave_incl_scan(v);
+                const uint32_t ex = inc - v;
                 if (g < c1) {
-                    RING[g] = ((cv + uint32_t(ex & 0xFFFFull)) << 8) | cc.cs;
+                    RING[g] = ((cv + (ex & 0x3FFu)) << 8) | cc.cs;
                     if (cc.act) {
-                        const uint32_t slot = c0 + ca + uint32_t(ex >> 32);
+                        const uint32_t slot = c0 + ca + (ex >> 19);
                         LIST[slot] = g;
-                        LIOFF[slot] = 3u * (ct + uint32_t((ex >> 16) & 0xFFFFull));
+                        LIOFF[slot] = 3u * (ct + ((ex >> 10) & 0x1FFu));
                     }
                 }
-                const unsigned long long tot = __shfl(inc, 63, 64);
-                cv += uint32_t(tot & 0xFFFFull);
-                ct += uint32_t((tot >> 16) & 0xFFFFull);
-                ca += uint32_t(tot >> 32);
+                const uint32_t tot = uw_wave_last(inc);
+                cv += tot & 0x3FFu;
+                ct += (tot >> 10) & 0x1FFu;
+                ca += tot >> 19;
             }
             if (lane == 0) {
                 wtot[wave][0] = cv;
@@ -419,7 +413,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 float r[8];
                 cell_corners(DENS, ns, L, x, y, z, r);
                 emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + LIOFF[slot], step, ox, oy,
-                          oz, TRI, RING, 0xFFFFFFFFu, owner_voff);
+                          oz, TRI, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
             }
             continue;
         }
@@ -429,7 +423,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         unsigned long long cnt = 0;  // nv | nt << 24 | act << 48 (per thread)
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
-            below |= uw_noise_planes<OCT>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
+            below |= uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
                      UW_BELOW_NOT_ABOVE;
             __syncthreads();
             for (int32_t g = xa * nn + tid; g < xe * nn; g += BLOCK) {
@@ -480,7 +474,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
             if (nslab > 1) {
-                uw_noise_planes<OCT>(a.f, T, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+                uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
                 __syncthreads();
             }
             uint32_t list_n = 0;
@@ -521,7 +515,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                         float r[8];
                         cell_corners(DENS, ns, L, x, y, z, r);
                         emit_cell(a, n, L, x, y, z, w & 255u, r, w >> 8, vo, io + LIOFF[e], step, ox, oy, oz, TRI,
-                                  RING, uint32_t(kRing - 1), owner_voff);
+                                  RING, uint32_t(kRing - 1), owner_voff, vo + nv, io + ni);
                     }
                     __syncthreads();
                     list_n = 0;
@@ -560,6 +554,7 @@ int32_t list_cap_for(const uw_batch_tuning& t) {
     int32_t cap = kMinList > t.block ? kMinList : t.block;
     for (int n = 1; n <= t.max_res; ++n)
         if (t.slab[n] == n && n * n * n <= kRing) cap = n * n * n > cap ? n * n * n : cap;
+    cap = (cap + 3) & ~3;  // keeps the LDS tables after the list 16-byte aligned
     return cap > kRing ? kRing : cap;
 }
 
@@ -606,6 +601,7 @@ struct BatchLaunch {
     BatchedKernel k;
     size_t lds;
     int grid;
+    size_t ztab_bytes;  // z tables of nchunks chunks
 };
 
 int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
@@ -619,6 +615,14 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         ctx->batch_order_cap = size_t(nchunks);
     }
     if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, sizeof(uint32_t)));
+    bl->ztab_bytes = size_t(nchunks) * uw_ztab_bytes(octaves, tune.max_res + 1);
+    if (bl->ztab_bytes > ctx->batch_ztab_bytes) {
+        (void)hipFree(ctx->d_batch_ztab);
+        ctx->d_batch_ztab = nullptr;
+        ctx->batch_ztab_bytes = 0;
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_ztab, bl->ztab_bytes));
+        ctx->batch_ztab_bytes = bl->ztab_bytes;
+    }
     bl->k = pick_kernel(tune.block, octaves);
     bl->lds = lds_fixed_bytes(size_t(list_cap_for(tune))) + lds_table_bytes(octaves) + dens_bytes_for(tune);
     UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(bl->k), hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -635,7 +639,7 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
 }
 
 int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, const int32_t* res, int32_t nchunks,
-                   const uw_mesh_out* out, uint32_t* head, uint32_t* order, hipStream_t st,
+                   const uw_mesh_out* out, uint32_t* head, uint32_t* order, unsigned char* ztab, hipStream_t st,
                    uw_vertex_tape* tape = nullptr) {
     const uw_batch_tuning& tune = ctx->batch_tuning;
     BatchArgs a;
@@ -658,8 +662,11 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
     a.order = order;
     a.head = head;
+    a.ztab = ztab;
+    a.zstride = tune.max_res + 1;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, head, a.totals);
     hipLaunchKernelGGL(k_order, dim3(1), dim3(1024), 0, st, res, nchunks, order);
+    if (int rc = uw_launch_ztables(ctx, a.f, coords, res, 0, nchunks, a.zstride, ztab, st)) return rc;
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
@@ -690,7 +697,8 @@ int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* r
         UW_HIP(ctx, hipGetLastError());
         return UW_OK;
     }
-    return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, st);
+    return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, ctx->d_batch_ztab,
+                          st);
 }
 
 int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
@@ -707,7 +715,8 @@ int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32
         UW_HIP(ctx, hipGetLastError());
         return UW_OK;
     }
-    return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, st, tape);
+    return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, ctx->d_batch_ztab,
+                          st, tape);
 }
 
 int uw_get_batch_tuning(const uw_ctx* ctx, uw_batch_tuning* t) {
@@ -731,9 +740,11 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     UW_HIP(ctx, hipSetDevice(ctx->device));
     BatchLaunch bl;
     if (int rc = prepare_batched(ctx, nchunks, &bl)) return rc;
-    // the graph owns its queue head and order buffers: the context's may be reallocated later
+    // the graph owns its queue head, order and z-table buffers: the context's may be reallocated
     uint32_t* gbuf = nullptr;
-    UW_HIP(ctx, hipMalloc(&gbuf, (size_t(nchunks) + 1) * sizeof(uint32_t)));
+    const size_t ord_bytes = ((size_t(nchunks) + 1) * sizeof(uint32_t) + 255) & ~size_t(255);
+    UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + bl.ztab_bytes));
+    unsigned char* gztab = reinterpret_cast<unsigned char*>(gbuf) + ord_bytes;
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(gbuf);
@@ -744,7 +755,7 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     int rc = UW_OK;
     const bool was_profiling = ctx->profiling;
     ctx->profiling = false;  // no timing events inside a graph
-    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gbuf + 1, cs);
+    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gbuf + 1, gztab, cs);
     ctx->profiling = was_profiling;
     hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &graph) : hipSuccess;
     (void)hipStreamDestroy(cs);

@@ -220,6 +220,7 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(s.voff);
     (void)hipFree(s.act);
     (void)hipFree(s.ioff);
+    (void)hipFree(s.ztab);
     (void)hipFree(ctx->d_coord1);
     (void)hipFree(ctx->d_verts1);
     (void)hipFree(ctx->d_inds1);
@@ -229,6 +230,7 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(ctx->d_cells1);
     (void)hipFree(ctx->d_batch_head);
     (void)hipFree(ctx->d_batch_order);
+    (void)hipFree(ctx->d_batch_ztab);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
     delete ctx;
 }

@@ -37,6 +37,8 @@ struct uw_plain_scratch {
     uint32_t* voff = nullptr;
     uint32_t* act = nullptr;   // (cell index) | (case << 24), compacted per chunk
     uint32_t* ioff = nullptr;  // index offset of each compacted active cell
+    unsigned char* ztab = nullptr;  // per-chunk z tables of the density kernel
+    size_t ztab_bytes = 0;
 };
 
 struct uw_graph {
@@ -56,6 +58,8 @@ struct uw_ctx {
     uint32_t* d_batch_head = nullptr;
     uint32_t* d_batch_order = nullptr;
     size_t batch_order_cap = 0;
+    unsigned char* d_batch_ztab = nullptr;  // per-chunk z tables of the eager batched path
+    size_t batch_ztab_bytes = 0;
     uint8_t perm[256];
     uw_tables host_tables;
     uw_tables* d_tables = nullptr;
@@ -106,3 +110,6 @@ struct uw_field {
 
 uw_field uw_make_field(const uw_params& p);
 int uw_check_resolution(uw_ctx* ctx, int32_t n);
+// Writes the per-chunk z tables (uw_ztab layout, `stride` samples per row) of a launch.
+int uw_launch_ztables(uw_ctx* ctx, const uw_field& f, const int32_t* coords, const int32_t* res, int32_t n_uniform,
+                      int32_t nchunks, int32_t stride, unsigned char* out, hipStream_t st);

@@ -123,25 +123,41 @@ struct uw_line_oct {
     float A0, B0, A1, B1;
 };
 
-// Per-line, per-octave setup for the lattice cell whose low/high z bytes are packed in zbits.
-// bb = b00 | b01 << 8 | b10 << 16 | b11 << 24 (second-level hashes of the line's x/y corners).
+// One z face of a line's lattice cell: A = lerp_y(lerp_x(P_..)), B = lerp_y(lerp_x(Gz_..)) for the
+// face whose z lattice byte is zb. bb = b00 | b01 << 8 | b10 << 16 | b11 << 24 (second-level
+// hashes of the line's x/y corners). The face depends on the z lattice index only through zb.
+__device__ __forceinline__ void uw_line_face(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zb,
+                                             const float4* GP, float* A, float* B) {
+    const uint32_t b00 = bb & 255u, b01 = (bb >> 8) & 255u, b10 = (bb >> 16) & 255u, b11 = bb >> 24;
+    const float4 G00 = GP[b00 ^ zb], G10 = GP[b10 ^ zb], G01 = GP[b01 ^ zb], G11 = GP[b11 ^ zb];
+    const float P00 = fmaf(G00.y, Y.d, G00.x * X.d), P10 = fmaf(G10.y, Y.d, G10.x * X.dm1);
+    const float P01 = fmaf(G01.y, Y.dm1, G01.x * X.d), P11 = fmaf(G11.y, Y.dm1, G11.x * X.dm1);
+    const float a = X.fade, b = Y.fade;
+    *A = uw_lerp(b, uw_lerp(a, P00, P10), uw_lerp(a, P01, P11));
+    *B = uw_lerp(b, uw_lerp(a, G00.z, G10.z), uw_lerp(a, G01.z, G11.z));
+}
+
+// Per-line, per-octave state for the lattice cell whose low/high z bytes are packed in zbits.
 __device__ __forceinline__ uw_line_oct uw_line_setup(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zbits,
                                                      const float4* GP) {
-    const uint32_t z0 = zbits & 255u, z1 = (zbits >> 8) & 255u;
-    const uint32_t b00 = bb & 255u, b01 = (bb >> 8) & 255u, b10 = (bb >> 16) & 255u, b11 = bb >> 24;
-    const float4 G000 = GP[b00 ^ z0], G100 = GP[b10 ^ z0], G010 = GP[b01 ^ z0], G110 = GP[b11 ^ z0];
-    const float4 G001 = GP[b00 ^ z1], G101 = GP[b10 ^ z1], G011 = GP[b01 ^ z1], G111 = GP[b11 ^ z1];
-    const float P000 = fmaf(G000.y, Y.d, G000.x * X.d), P100 = fmaf(G100.y, Y.d, G100.x * X.dm1);
-    const float P010 = fmaf(G010.y, Y.dm1, G010.x * X.d), P110 = fmaf(G110.y, Y.dm1, G110.x * X.dm1);
-    const float P001 = fmaf(G001.y, Y.d, G001.x * X.d), P101 = fmaf(G101.y, Y.d, G101.x * X.dm1);
-    const float P011 = fmaf(G011.y, Y.dm1, G011.x * X.d), P111 = fmaf(G111.y, Y.dm1, G111.x * X.dm1);
-    const float a = X.fade, b = Y.fade;
     uw_line_oct s;
-    s.A0 = uw_lerp(b, uw_lerp(a, P000, P100), uw_lerp(a, P010, P110));
-    s.A1 = uw_lerp(b, uw_lerp(a, P001, P101), uw_lerp(a, P011, P111));
-    s.B0 = uw_lerp(b, uw_lerp(a, G000.z, G100.z), uw_lerp(a, G010.z, G110.z));
-    s.B1 = uw_lerp(b, uw_lerp(a, G001.z, G101.z), uw_lerp(a, G011.z, G111.z));
+    uw_line_face(X, Y, bb, zbits & 255u, GP, &s.A0, &s.B0);
+    uw_line_face(X, Y, bb, (zbits >> 8) & 255u, GP, &s.A1, &s.B1);
     return s;
+}
+
+// Moves a line's state to the cell whose packed z bytes are zbits. When the new cell's low face
+// has the previous cell's high-face byte (the line stepped to the next cell, modulo 256, which
+// is all the noise sees of the index), that face is reused and only the new high face is built.
+__device__ __forceinline__ void uw_line_advance(uw_line_oct& s, uint32_t prev, const uw_axis& X, const uw_axis& Y,
+                                                uint32_t bb, uint32_t zbits, const float4* GP) {
+    if (prev != 0xFFFFFFFFu && (zbits & 255u) == ((prev >> 8) & 255u)) {
+        s.A0 = s.A1;
+        s.B0 = s.B1;
+        uw_line_face(X, Y, bb, (zbits >> 8) & 255u, GP, &s.A1, &s.B1);
+    } else {
+        s = uw_line_setup(X, Y, bb, zbits, GP);
+    }
 }
 
 // Unscaled octave value at the line sample, clamped to the unscaled image of [-1, 1]; the caller
@@ -178,6 +194,75 @@ __device__ __forceinline__ void uw_build_chunk_tables(const uw_field& f, const u
     for (int i = tid; i < L; i += nthr) uw_bias_terms(f, uw_sample_coord(f, 2, cz, i, step), &T.ADJ[i], &T.POST[i]);
 }
 
+// The x and y tables only (the density loops read the z entries from the chunk's z table in HBM,
+// see uw_ztab): AX[k L + i] hashed, AY[k L + i].
+__device__ __forceinline__ void uw_build_xy_tables(const uw_field& f, const uint8_t* perm, int32_t cx, int32_t cy,
+                                                   int32_t L, float step, uw_axis* AX, uw_axis* AY, int tid,
+                                                   int nthr) {
+    const int32_t O = f.octaves;
+    for (int e = tid; e < 2 * O * L; e += nthr) {
+        const bool is_y = e >= O * L;
+        const int32_t r = is_y ? e - O * L : e;
+        const int32_t k = r / L, i = r - k * L;
+        const double fk = f.octave_freq[k];
+        if (is_y) AY[r] = uw_axis_of(uw_sample_coord(f, 1, cy, i, step), fk);
+        else AX[r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, 0, cx, i, step), fk), perm);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-chunk z tables in HBM. Every lane of a wave walks its line at the same z index, so the z
+// entries and the bias split of a sample are wave-uniform: kept in HBM (written by a previous
+// kernel, uw_launch_ztables) they are fetched with scalar loads into scalar registers, costing
+// neither vector instructions nor vector registers. Layout per chunk, `stride` entries per row:
+//   az[k * stride + i]  (uw_axis, octave k, sample i), then adj[stride], post[stride] floats.
+struct uw_ztab {
+    const uw_axis* az;
+    const float* adj;
+    const float* post;
+    int32_t stride;
+};
+
+__host__ __device__ inline size_t uw_ztab_bytes(int32_t octaves, int32_t stride) {
+    return (size_t(octaves) * size_t(stride) * sizeof(uw_axis) + 2 * size_t(stride) * sizeof(float) + 15) & ~size_t(15);
+}
+
+__device__ __forceinline__ uw_ztab uw_ztab_of(const unsigned char* base, int32_t chunk, int32_t octaves,
+                                              int32_t stride) {
+    uw_ztab t;
+    const unsigned char* p = base + size_t(chunk) * uw_ztab_bytes(octaves, stride);
+    t.az = reinterpret_cast<const uw_axis*>(p);
+    t.adj = reinterpret_cast<const float*>(t.az + size_t(octaves) * stride);
+    t.post = t.adj + stride;
+    t.stride = stride;
+    return t;
+}
+
+// Scalar (constant address space) loads of wave-uniform HBM data written by an earlier kernel.
+typedef const __attribute__((address_space(4))) uint32_t* uw_const_u32_ptr;
+__device__ __forceinline__ float uw_sload(const float* p) { return __uint_as_float(*(uw_const_u32_ptr)(p)); }
+__device__ __forceinline__ uw_axis uw_sload(const uw_axis* p) {
+    const uw_const_u32_ptr q = (uw_const_u32_ptr)(p);
+    uw_axis a;
+    a.bits = q[0];
+    a.d = __uint_as_float(q[1]);
+    a.dm1 = __uint_as_float(q[2]);
+    a.fade = __uint_as_float(q[3]);
+    return a;
+}
+
+// One chunk's z table (chunk coordinate cz, L samples per axis).
+__device__ __forceinline__ void uw_fill_ztab(const uw_field& f, int32_t cz, int32_t L, float step, uw_axis* az,
+                                             float* adj, float* post, int32_t stride, int tid, int nthr) {
+    const int32_t O = f.octaves;
+    for (int e = tid; e < (O + 1) * L; e += nthr) {
+        const int32_t k = e / L, i = e - k * L;
+        const double pz = uw_sample_coord(f, 2, cz, i, step);
+        if (k < O) az[k * stride + i] = uw_axis_of(pz, f.octave_freq[k]);
+        else uw_bias_terms(f, pz, &adj[i], &post[i]);
+    }
+}
+
 // Sign summary bits of a set of samples: some sample not above iso (so not early-blank), some
 // below iso, some at or above iso (a cell can straddle iso only if both of the last two hold).
 #define UW_BELOW_NOT_ABOVE 1
@@ -198,9 +283,9 @@ struct uw_octaves {
 // Plane x is written to dst + slot(x) * L * L with slot(x) = x % nslot, z fastest.
 // Returns, per thread, the OR of the UW_BELOW_* bits of its samples.
 template <int OCT>
-__device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uint8_t* P,
-                                               const float4* GP, int32_t L, int32_t xa, int32_t xb, float* dst,
-                                               int32_t nslot, int tid, int nthr) {
+__device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uw_ztab& Zt,
+                                               const uint8_t* P, const float4* GP, int32_t L, int32_t xa, int32_t xb,
+                                               float* dst, int32_t nslot, int tid, int nthr) {
     constexpr int OMAX = uw_octaves<OCT>::kMax;
     const int32_t O = uw_octaves<OCT>::count(f);
     const int32_t lines = (xb - xa) * L;
@@ -225,27 +310,28 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
         // The z entries are wave-uniform: those of sample iz + 1 are fetched while sample iz is
         // evaluated (the last fetch repeats index L - 1 and is discarded).
         uw_axis Zn[OMAX];
-        float adjn = T.ADJ[0], postn = T.POST[0];
+        float adjn = uw_sload(Zt.adj), postn = uw_sload(Zt.post);
 #pragma unroll
         for (int k = 0; k < OMAX; ++k)
-            if (k < O) Zn[k] = T.AZ[k * L];
+            if (k < O) Zn[k] = uw_sload(Zt.az + k * Zt.stride);
         for (int32_t iz = 0; iz < L; ++iz) {
             uw_axis Z[OMAX];
 #pragma unroll
-            for (int k = 0; k < OMAX; ++k) Z[k] = Zn[k];
+            for (int k = 0; k < OMAX; ++k)
+                if (k < O) Z[k] = Zn[k];
             const float adj = adjn, post = postn;
             const int32_t izn = iz + 1 < L ? iz + 1 : iz;
 #pragma unroll
             for (int k = 0; k < OMAX; ++k)
-                if (k < O) Zn[k] = T.AZ[k * L + izn];
-            adjn = T.ADJ[izn];
-            postn = T.POST[izn];
+                if (k < O) Zn[k] = uw_sload(Zt.az + k * Zt.stride + izn);
+            adjn = uw_sload(Zt.adj + izn);
+            postn = uw_sload(Zt.post + izn);
             float total = 0.0f;
 #pragma unroll
             for (int k = 0; k < OMAX; ++k) {
                 if (k < O) {
                     if (Z[k].bits != zcur[k]) {
-                        st[k] = uw_line_setup(T.AX[k * L + ix], T.AY[k * L + iy], bb[k], Z[k].bits, GP);
+                        uw_line_advance(st[k], zcur[k], T.AX[k * L + ix], T.AY[k * L + iy], bb[k], Z[k].bits, GP);
                         zcur[k] = Z[k].bits;
                     }
                     total = fmaf(f.weight[k], uw_line_sample(st[k], Z[k], f.clamp_lo, f.clamp_hi), total);

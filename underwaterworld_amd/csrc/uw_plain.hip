@@ -15,6 +15,7 @@
 //   ioff  [c][C] u32      index offset of each compacted active cell
 #include "uw_mesh.h"
 #include "uw_noise.h"
+#include "uw_wave.h"
 
 namespace {
 
@@ -33,21 +34,18 @@ template <int OCT>
 __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_tables* __restrict__ tab,
                                                          const int32_t* __restrict__ coords, int32_t n,
                                                          float step, int32_t planes, int32_t slabs,
+                                                         const unsigned char* __restrict__ ztab,
                                                          float* __restrict__ dens, uint32_t* __restrict__ below) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int32_t L = n + 1, O = f.octaves;
-    const int32_t Lp = (L + 3) & ~3;
     const int32_t chunk = blockIdx.x / slabs, slab = blockIdx.x % slabs;
     const int32_t x0 = slab * planes;
     const int32_t nx = min(planes, L - x0);
     float4* GP = reinterpret_cast<float4*>(smem);                  // 256 x 16 B
-    float* ADJ = reinterpret_cast<float*>(GP + 256);               // Lp
-    float* POST = ADJ + Lp;                                        // Lp
-    uint8_t* P = reinterpret_cast<uint8_t*>(POST + Lp);            // 256 B
-    uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);             // [O][L] x3
+    uint8_t* P = reinterpret_cast<uint8_t*>(GP + 256);             // 256 B
+    uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);             // [O][L] x2
     uw_axis* AY = AX + O * L;
-    uw_axis* AZ = AY + O * L;
-    float* SLAB = reinterpret_cast<float*>(AZ + O * L);            // [planes][L][L]
+    float* SLAB = reinterpret_cast<float*>(AY + O * L);            // [planes][L][L]
 
     __shared__ uint32_t s_bits;
     const int32_t cx = coords[3 * chunk], cy = coords[3 * chunk + 1], cz = coords[3 * chunk + 2];
@@ -58,11 +56,13 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
         GP[i] = tab->grad[p & 15];
     }
     __syncthreads();
-    const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
-    uw_build_chunk_tables(f, P, cx, cy, cz, L, step, T, threadIdx.x, blockDim.x);
+    (void)cz;
+    const uw_chunk_tables T = {AX, AY, nullptr, nullptr, nullptr};
+    uw_build_xy_tables(f, P, cx, cy, L, step, AX, AY, threadIdx.x, blockDim.x);
+    const uw_ztab Zt = uw_ztab_of(ztab, chunk, O, L);
     __syncthreads();
     // plane x lands in slot x % planes = x - x0
-    const int bits = uw_noise_planes<OCT>(f, T, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
+    const int bits = uw_noise_planes<OCT>(f, T, Zt, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
     if (bits) atomicOr(&s_bits, uint32_t(bits));
     __syncthreads();
     const int32_t LL = L * L;
@@ -76,12 +76,7 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
 // Block-wide exclusive scan of one u32 per thread (kThreads threads, wave64).
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total, uint32_t* lds_waves) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(inc, d, 64);
-        if (lane >= d) inc += t;
-    }
+    const uint32_t inc = uw_wave_incl_scan(v);
     if (lane == 63) lds_waves[wave] = inc;
     __syncthreads();
     uint32_t wave_base = 0, sum = 0;
@@ -170,14 +165,8 @@ __global__ void __launch_bounds__(1024) k_offsets(uw_chunk_info* __restrict__ in
         if (c < nchunks) r = info[c];
         const unsigned long long v = r.n_verts, i = r.n_inds;
         const unsigned int nb = (c < nchunks && r.n_inds > 0) ? 1u : 0u;
-        unsigned long long sv = v, si = i;
-        unsigned int sn = nb;
-#pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const unsigned long long tv = __shfl_up(sv, d, 64), ti = __shfl_up(si, d, 64);
-            const unsigned int tn = __shfl_up(sn, d, 64);
-            if (lane >= d) { sv += tv; si += ti; sn += tn; }
-        }
+        const unsigned long long sv = uw_wave_incl_scan(v), si = uw_wave_incl_scan(i);
+        const unsigned int sn = uw_wave_incl_scan(nb);
         if (lane == 63) { wv[wave] = sv; wi[wave] = si; wn[wave] = sn; }
         __syncthreads();
         unsigned long long bv = 0, bi = 0, tv = 0, ti = 0;
@@ -249,6 +238,8 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
         const uint32_t myv = voff[cbase + cell];
         const uint64_t vb = ci.vert_off + myv;
         const uint64_t ib = ci.ind_off + ioff[cbase + a];
+        // ends of the chunk's output ranges: writes stay inside them whatever the ids
+        const uint64_t vlim = ci.vert_off + ci.n_verts, ilim = ci.ind_off + ci.n_inds;
         const bool coll = eo.tri_normals || eo.tri_cells;
         float q0[3] = {0.f, 0.f, 0.f}, q1[3] = {0.f, 0.f, 0.f};
         uint32_t seen = 0;
@@ -262,7 +253,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
             if (owned & bit) {
                 const uint32_t rank = __popc(tab->before[cs][e] & owned);
                 id = myv + rank;
-                if (fresh) {
+                if (fresh && vb + rank < vlim) {
                     float* dst = verts + eo.vstride * (vb + rank);
                     dst[0] = p[0];
                     dst[1] = p[1];
@@ -278,6 +269,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                 id = voff[cbase + q] + __popc(tab->before[qc][qe] & uw_owned_mask(qx, qy, qz));
             }
             seen |= bit;
+            if (ib + j >= ilim) continue;
             uw_store_index(inds, eo.ind16, ib + j, id);
             if (coll) {
                 const int k = j % 3;
@@ -305,8 +297,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
 
 // LDS bytes of k_noise_grid.
 size_t noise_lds_bytes(int32_t O, int32_t planes, int32_t L) {
-    const size_t Lp = size_t((L + 3) & ~3);
-    return 256 * sizeof(float4) + 2 * sizeof(float) * Lp + 256 + sizeof(uw_axis) * 3 * size_t(O) * size_t(L) +
+    return 256 * sizeof(float4) + 256 + sizeof(uw_axis) * 2 * size_t(O) * size_t(L) +
            sizeof(float) * size_t(planes) * size_t(L) * size_t(L);
 }
 
@@ -316,8 +307,8 @@ int32_t noise_planes(int32_t L) {
     return p < 1 ? 1 : (p > L ? L : p);
 }
 
-using NoiseKernel = void (*)(uw_field, const uw_tables*, const int32_t*, int32_t, float, int32_t, int32_t, float*,
-                             uint32_t*);
+using NoiseKernel = void (*)(uw_field, const uw_tables*, const int32_t*, int32_t, float, int32_t, int32_t,
+                             const unsigned char*, float*, uint32_t*);
 
 NoiseKernel pick_noise(int octaves) {
     switch (octaves) {
@@ -366,10 +357,20 @@ int launch_noise(uw_ctx* ctx, const uw_field& f, const int32_t* coords, int32_t 
     NoiseKernel k = pick_noise(f.octaves);
     UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(lds)));
+    uw_plain_scratch& s = ctx->plain;
+    const size_t zbytes = size_t(nchunks) * uw_ztab_bytes(f.octaves, L);
+    if (s.ztab_bytes < zbytes) {
+        (void)hipFree(s.ztab);
+        s.ztab = nullptr;
+        s.ztab_bytes = 0;
+        UW_HIP(ctx, hipMalloc(&s.ztab, zbytes));
+        s.ztab_bytes = zbytes;
+    }
+    if (int rc = uw_launch_ztables(ctx, f, coords, nullptr, n, nchunks, L, s.ztab, st)) return rc;
     hipLaunchKernelGGL(k_fill_u32, dim3(uint32_t((nchunks + 255) / 256)), dim3(256), 0, st, below, uint64_t(nchunks), 0u);
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(k, dim3(uint32_t(blocks)), dim3(kThreads), lds, st, f, ctx->d_tables, coords, n, step, planes,
-                       slabs, dens, below);
+                       slabs, s.ztab, dens, below);
     if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
