@@ -114,9 +114,10 @@ struct CellCounts {
 };
 
 // Corner densities of cell (x, y, z) from the density-plane ring (plane x at slot x % ns).
-__device__ __forceinline__ void cell_corners(const float* ring, int32_t ns, int32_t L, int32_t x, int32_t y,
+__device__ __forceinline__ void cell_corners(const float* ring, uw_divider dns, int32_t L, int32_t x, int32_t y,
                                              int32_t z, float r[8]) {
-    const int32_t s0 = x % ns;
+    const int32_t ns = int32_t(dns.d);
+    const int32_t s0 = int32_t(uw_mod(uint32_t(x), dns));
     const int32_t s1 = (s0 + 1 == ns) ? 0 : s0 + 1;
     const float* p0 = ring + s0 * L * L + y * L + z;
     const float* p1 = ring + s1 * L * L + y * L + z;
@@ -130,10 +131,10 @@ __device__ __forceinline__ void cell_corners(const float* ring, int32_t ns, int3
     r[7] = p0[L + 1];
 }
 
-__device__ __forceinline__ CellCounts classify_cell(const float* ring, int32_t ns, int32_t L, int32_t x, int32_t y,
-                                                    int32_t z, float iso, const uint32_t* lut) {
+__device__ __forceinline__ CellCounts classify_cell(const float* ring, uw_divider dns, int32_t L, int32_t x,
+                                                    int32_t y, int32_t z, float iso, const uint32_t* lut) {
     float r[8];
-    cell_corners(ring, ns, L, x, y, z, r);
+    cell_corners(ring, dns, L, x, y, z, r);
     CellCounts c;
     c.cs = uw_case_of(r, iso);
     const uint32_t w = lut[c.cs];
@@ -145,8 +146,8 @@ __device__ __forceinline__ CellCounts classify_cell(const float* ring, int32_t n
 
 // LDS carve-up (bytes): fixed part, then the per-octave axis tables, then the density ring.
 __host__ __device__ constexpr size_t lds_fixed_bytes(size_t list_cap) {
-    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + size_t(kRing) * 4 + 2 * list_cap * 4 +
-           256 /*P*/;
+    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + 256 * 12 * 2 /*BEF*/ + size_t(kRing) * 4 +
+           2 * list_cap * 4 + 256 /*P*/;
 }
 __host__ __device__ inline size_t lds_table_bytes(int octaves) { return 2 * size_t(octaves) * kMaxL * sizeof(uw_axis); }
 
@@ -158,10 +159,9 @@ template <class OwnerVoff>
 __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t L, int32_t x, int32_t y, int32_t z,
                                           uint32_t cs, const float r[8], uint32_t myv, uint64_t vo, uint64_t ib,
                                           float step, float ox, float oy, float oz, const uint64_t* TRI,
-                                          const uint32_t* RING, uint32_t ring_mask, OwnerVoff owner_voff,
-                                          uint64_t vlim, uint64_t ilim) {
+                                          const uint16_t* BEF, const uint32_t* RING, uint32_t ring_mask,
+                                          OwnerVoff owner_voff, uint64_t vlim, uint64_t ilim) {
     const float iso = a.f.iso;
-    const uw_tables* tab = a.tab;
     const uint32_t owned = uw_owned_mask(x, y, z);
     const uint64_t row = TRI[cs];
     const int nidx = 3 * int(row & 15u);
@@ -178,7 +178,7 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
         if (fresh || coll) uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
         uint32_t id;
         if (owned & bit) {
-            const uint32_t rank = __popc(uint32_t(tab->before[cs][ed]) & owned);
+            const uint32_t rank = __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
             id = myv + rank;
             // vlim / ilim end the chunk's reserved ranges: writes stay inside them whatever the ids
             if (fresh && vb + rank < vlim) {
@@ -206,7 +206,7 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
             uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
             const uint32_t q = uint32_t((qx * n + qy) * n + qz);
             const uint32_t qw = RING[q & ring_mask];
-            id = owner_voff(q, qw) + __popc(uint32_t(tab->before[qw & 255u][qe]) & uw_owned_mask(qx, qy, qz));
+            id = owner_voff(q, qw) + __popc(uint32_t(BEF[(qw & 255u) * 12 + qe]) & uw_owned_mask(qx, qy, qz));
         }
         seen |= bit;
         if (ib + j >= ilim) continue;
@@ -242,7 +242,8 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     float4* GP = reinterpret_cast<float4*>(smem);
     uint64_t* TRI = reinterpret_cast<uint64_t*>(GP + 256);
     uint32_t* LUT = reinterpret_cast<uint32_t*>(TRI + 256);
-    uint32_t* RING = LUT + 256;
+    uint16_t* BEF = reinterpret_cast<uint16_t*>(LUT + 256);  // first-seen rank masks [256][12]
+    uint32_t* RING = reinterpret_cast<uint32_t*>(BEF + 256 * 12);
     uint32_t* LIST = RING + kRing;
     uint32_t* LIOFF = LIST + a.list_cap;
     uint8_t* P = reinterpret_cast<uint8_t*>(LIOFF + a.list_cap);  // list_cap is a multiple of 4
@@ -264,6 +265,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         TRI[i] = tab->tri[i];
         LUT[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
     }
+    for (int i = tid; i < 256 * 12; i += BLOCK) BEF[i] = (&tab->before[0][0])[i];
     const float iso = a.f.iso;
 
     for (;;) {
@@ -275,8 +277,9 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         __syncthreads();
         const uint32_t task = s_task;
         if (task >= uint32_t(a.nchunks)) break;
-        const int32_t c = int32_t(a.order[task]);
-        const int32_t n = a.res[c];
+        // chunk-level values are wave-uniform: keep them in scalar registers
+        const int32_t c = __builtin_amdgcn_readfirstlane(int32_t(a.order[task]));
+        const int32_t n = __builtin_amdgcn_readfirstlane(a.res[c]);
         if (n < 1 || n > a.max_res || !uw_octaves<OCT>::accepts(O)) {
             if (tid == 0) {
                 uw_chunk_info r = {UW_CHUNK_INVALID | UW_CHUNK_BLANK, 0, 0, 0, 0, 0};
@@ -286,17 +289,20 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         }
         const int32_t L = n + 1, nn = n * n, C = nn * n;
         const uw_divider dn = uw_make_divider(uint32_t(n)), dnn = uw_make_divider(uint32_t(nn));
-        const int32_t cx = a.coords[3 * c], cy = a.coords[3 * c + 1], cz = a.coords[3 * c + 2];
+        const int32_t cx = __builtin_amdgcn_readfirstlane(a.coords[3 * c]);
+        const int32_t cy = __builtin_amdgcn_readfirstlane(a.coords[3 * c + 1]);
+        const int32_t cz = __builtin_amdgcn_readfirstlane(a.coords[3 * c + 2]);
         const float step = float(a.f.extent) / float(n);
         const float ox = float(a.f.extent * cx), oy = float(a.f.extent * cy), oz = float(a.f.extent * cz);
         const uw_chunk_tables T = {AX, AY, nullptr, nullptr, nullptr};
         uw_build_xy_tables(a.f, P, cx, cy, L, step, AX, AY, tid, BLOCK);
-        const uw_ztab Zt = uw_ztab_of(a.ztab, __builtin_amdgcn_readfirstlane(c), O, a.zstride);
+        const uw_ztab Zt = uw_ztab_of(a.ztab, c, O, a.zstride);
         __syncthreads();
 
         const int32_t sx = a.sx[n];
         const int32_t ns = sx + 1;
         const int32_t nslab = (n + sx - 1) / sx;
+        const uw_divider dns = uw_make_divider(uint32_t(ns));
 
         if (nslab == 1 && C <= a.list_cap) {
             // ---------- one density evaluation, one classification pass ----------
@@ -324,7 +330,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 if (g < c1) {
                     int32_t x, y, z;
                     uw_cell_xyz(g, dn, dnn, &x, &y, &z);
-                    cc = classify_cell(DENS, ns, L, x, y, z, iso, LUT);
+                    cc = classify_cell(DENS, dns, L, x, y, z, iso, LUT);
                 }
                 // per wave of 64 cells: vertices <= 768 (10 bits), triangles <= 320 (9 bits),
                 // active cells <= 64 (7 bits), so one 32-bit scan carries all three
@@ -411,9 +417,9 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 int32_t x, y, z;
                 uw_cell_xyz(g, dn, dnn, &x, &y, &z);
                 float r[8];
-                cell_corners(DENS, ns, L, x, y, z, r);
+                cell_corners(DENS, dns, L, x, y, z, r);
                 emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + LIOFF[slot], step, ox, oy,
-                          oz, TRI, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
+                          oz, TRI, BEF, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
             }
             continue;
         }
@@ -429,7 +435,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
             for (int32_t g = xa * nn + tid; g < xe * nn; g += BLOCK) {
                 int32_t x, y, z;
                 uw_cell_xyz(uint32_t(g), dn, dnn, &x, &y, &z);
-                const CellCounts cc = classify_cell(DENS, ns, L, x, y, z, iso, LUT);
+                const CellCounts cc = classify_cell(DENS, dns, L, x, y, z, iso, LUT);
                 cnt += (unsigned long long)cc.nv | ((unsigned long long)cc.nt << 24) |
                        ((unsigned long long)cc.act << 48);
             }
@@ -486,7 +492,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 if (g < gend) {
                     int32_t x, y, z;
                     uw_cell_xyz(uint32_t(g), dn, dnn, &x, &y, &z);
-                    cc = classify_cell(DENS, ns, L, x, y, z, iso, LUT);
+                    cc = classify_cell(DENS, dns, L, x, y, z, iso, LUT);
                 }
                 const unsigned long long packed = (unsigned long long)cc.nv | ((unsigned long long)cc.nt << 16) |
                                                   ((unsigned long long)cc.act << 32);
@@ -513,9 +519,9 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                         int32_t x, y, z;
                         uw_cell_xyz(gg, dn, dnn, &x, &y, &z);
                         float r[8];
-                        cell_corners(DENS, ns, L, x, y, z, r);
+                        cell_corners(DENS, dns, L, x, y, z, r);
                         emit_cell(a, n, L, x, y, z, w & 255u, r, w >> 8, vo, io + LIOFF[e], step, ox, oy, oz, TRI,
-                                  RING, uint32_t(kRing - 1), owner_voff, vo + nv, io + ni);
+                                  BEF, RING, uint32_t(kRing - 1), owner_voff, vo + nv, io + ni);
                     }
                     __syncthreads();
                     list_n = 0;

@@ -9,6 +9,7 @@
 
 #include "../../include/uwmesh.h"
 #include "mc_table.h"
+#include "uw_div.h"
 
 // Device-resident lookup tables, uploaded once per context.
 struct uw_tables {

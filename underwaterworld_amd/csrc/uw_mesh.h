@@ -65,18 +65,6 @@ __device__ __forceinline__ void uw_edge_owner(int32_t x, int32_t y, int32_t z, u
     }
 }
 
-// Division by a small runtime divisor d via a 32-bit reciprocal: exact while g * d < 2^32.
-struct uw_divider {
-    uint32_t d, m;
-};
-__host__ __device__ __forceinline__ uw_divider uw_make_divider(uint32_t d) {
-    uw_divider r;
-    r.d = d;
-    r.m = uint32_t((0x100000000ull + d - 1) / d);
-    return r;
-}
-__device__ __forceinline__ uint32_t uw_div(uint32_t g, uw_divider v) { return v.d == 1u ? g : __umulhi(g, v.m); }
-
 // Decode a chunk-local cell index g = (x n + y) n + z.
 __device__ __forceinline__ void uw_cell_xyz(uint32_t g, uw_divider dn, uw_divider dnn, int32_t* x, int32_t* y,
                                             int32_t* z) {

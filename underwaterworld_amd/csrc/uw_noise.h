@@ -200,10 +200,11 @@ __device__ __forceinline__ void uw_build_xy_tables(const uw_field& f, const uint
                                                    int32_t L, float step, uw_axis* AX, uw_axis* AY, int tid,
                                                    int nthr) {
     const int32_t O = f.octaves;
+    const uw_divider dL = uw_make_divider(uint32_t(L));
     for (int e = tid; e < 2 * O * L; e += nthr) {
         const bool is_y = e >= O * L;
         const int32_t r = is_y ? e - O * L : e;
-        const int32_t k = r / L, i = r - k * L;
+        const int32_t k = int32_t(uw_div(uint32_t(r), dL)), i = r - k * L;
         const double fk = f.octave_freq[k];
         if (is_y) AY[r] = uw_axis_of(uw_sample_coord(f, 1, cy, i, step), fk);
         else AX[r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, 0, cx, i, step), fk), perm);
@@ -255,8 +256,9 @@ __device__ __forceinline__ uw_axis uw_sload(const uw_axis* p) {
 __device__ __forceinline__ void uw_fill_ztab(const uw_field& f, int32_t cz, int32_t L, float step, uw_axis* az,
                                              float* adj, float* post, int32_t stride, int tid, int nthr) {
     const int32_t O = f.octaves;
+    const uw_divider dL = uw_make_divider(uint32_t(L));
     for (int e = tid; e < (O + 1) * L; e += nthr) {
-        const int32_t k = e / L, i = e - k * L;
+        const int32_t k = int32_t(uw_div(uint32_t(e), dL)), i = e - k * L;
         const double pz = uw_sample_coord(f, 2, cz, i, step);
         if (k < O) az[k * stride + i] = uw_axis_of(pz, f.octave_freq[k]);
         else uw_bias_terms(f, pz, &adj[i], &post[i]);
@@ -289,11 +291,13 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
     constexpr int OMAX = uw_octaves<OCT>::kMax;
     const int32_t O = uw_octaves<OCT>::count(f);
     const int32_t lines = (xb - xa) * L;
+    const uw_divider dL = uw_make_divider(uint32_t(L)), dslot = uw_make_divider(uint32_t(nslot));
     // running extremes of the thread's samples; the sign bits follow from them at the end
     float vmin = INFINITY, vmax = -INFINITY;
     for (int32_t t = tid; t < lines; t += nthr) {
-        const int32_t ix = xa + t / L, iy = t - (t / L) * L;
-        float* out = dst + (ix % nslot) * L * L + iy * L;
+        const int32_t tq = int32_t(uw_div(uint32_t(t), dL));
+        const int32_t ix = xa + tq, iy = t - tq * L;
+        float* out = dst + int32_t(uw_mod(uint32_t(ix), dslot)) * L * L + iy * L;
         uint32_t bb[OMAX];
         uw_line_oct st[OMAX];
         uint32_t zcur[OMAX];

@@ -109,6 +109,7 @@ __global__ void __launch_bounds__(kThreads) k_classify(float iso, const uw_table
     for (int i = threadIdx.x; i < 256; i += blockDim.x) lut[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
     __syncthreads();
     const int32_t L = n + 1, C = n * n * n, nn = n * n;
+    const uw_divider dn = uw_make_divider(uint32_t(n)), dnn = uw_make_divider(uint32_t(nn));
     const float* D = dens + uint64_t(chunk) * uint64_t(L) * uint64_t(L) * uint64_t(L);
     const uint64_t cbase = uint64_t(chunk) * uint64_t(C);
     uint32_t carry_v = 0, carry_i = 0, carry_a = 0;
@@ -116,7 +117,8 @@ __global__ void __launch_bounds__(kThreads) k_classify(float iso, const uw_table
         const int32_t cell = r0 + threadIdx.x;
         uint32_t packed = 0, cs = 0;
         if (cell < C) {
-            const int32_t x = cell / nn, y = (cell / n) % n, z = cell % n;
+            int32_t x, y, z;
+        uw_cell_xyz(uint32_t(cell), dn, dnn, &x, &y, &z);
             float r[8];
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -217,15 +219,22 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
     const uw_chunk_info ci = info[chunk];
     if (ci.flags & (UW_CHUNK_BLANK | UW_CHUNK_OVERFLOW)) return;
     const int32_t L = n + 1, C = n * n * n, nn = n * n;
+    const uw_divider dn = uw_make_divider(uint32_t(n)), dnn = uw_make_divider(uint32_t(nn));
     const float* D = dens + uint64_t(chunk) * uint64_t(L) * uint64_t(L) * uint64_t(L);
     const uint64_t cbase = uint64_t(chunk) * uint64_t(C);
     const float ox = float(extent * coords[3 * chunk]), oy = float(extent * coords[3 * chunk + 1]),
                 oz = float(extent * coords[3 * chunk + 2]);
+    __shared__ uint64_t s_tri[256];
+    __shared__ uint16_t s_bef[256 * 12];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_tri[i] = tab->tri[i];
+    for (int i = threadIdx.x; i < 256 * 12; i += blockDim.x) s_bef[i] = (&tab->before[0][0])[i];
+    __syncthreads();
     for (uint32_t a = threadIdx.x; a < ci.n_active; a += blockDim.x) {
         const uint32_t ent = act[cbase + a];
         const int32_t cell = int32_t(ent & 0xFFFFFFu);
         const uint32_t cs = ent >> 24;
-        const int32_t x = cell / nn, y = (cell / n) % n, z = cell % n;
+        int32_t x, y, z;
+        uw_cell_xyz(uint32_t(cell), dn, dnn, &x, &y, &z);
         float r[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -233,7 +242,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
             r[i] = D[(x + int32_t(o & 1u)) * L * L + (y + int32_t((o >> 1) & 1u)) * L + z + int32_t((o >> 2) & 1u)];
         }
         const uint32_t owned = uw_owned_mask(x, y, z);
-        const uint64_t row = tab->tri[cs];
+        const uint64_t row = s_tri[cs];
         const int nidx = 3 * int(row & 15u);
         const uint32_t myv = voff[cbase + cell];
         const uint64_t vb = ci.vert_off + myv;
@@ -251,7 +260,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
             if (fresh || coll) uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
             uint32_t id;
             if (owned & bit) {
-                const uint32_t rank = __popc(tab->before[cs][e] & owned);
+                const uint32_t rank = __popc(uint32_t(s_bef[cs * 12 + e]) & owned);
                 id = myv + rank;
                 if (fresh && vb + rank < vlim) {
                     float* dst = verts + eo.vstride * (vb + rank);
@@ -266,7 +275,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                 uw_edge_owner(x, y, z, e, &qx, &qy, &qz, &qe);
                 const int32_t q = (qx * n + qy) * n + qz;
                 const uint32_t qc = cases[cbase + q];
-                id = voff[cbase + q] + __popc(tab->before[qc][qe] & uw_owned_mask(qx, qy, qz));
+                id = voff[cbase + q] + __popc(uint32_t(s_bef[qc * 12 + qe]) & uw_owned_mask(qx, qy, qz));
             }
             seen |= bit;
             if (ib + j >= ilim) continue;
