@@ -50,6 +50,9 @@ struct BatchArgs {
     uint64_t vcap, icap;
     int32_t max_res;      // largest resolution accepted by this launch (sizes the LDS)
     int32_t list_cap;     // entries of the compacted active-cell list
+    int32_t ring_cap;     // entries of the cell ring: kRing, or list_cap when every chunk is one-pass
+    int32_t has_lioff;    // 1: a separate index-offset list follows the active list (x-slab chunks)
+    int32_t axis_stride;  // entries per octave row of the x / y tables: max_res + 1
     int8_t sx[kMaxN + 1]; // cell planes per slab for each resolution
 };
 
@@ -145,11 +148,13 @@ __device__ __forceinline__ CellCounts classify_cell(const float* ring, uw_divide
 }
 
 // LDS carve-up (bytes): fixed part, then the per-octave axis tables, then the density ring.
-__host__ __device__ constexpr size_t lds_fixed_bytes(size_t list_cap) {
-    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + 256 * 12 * 2 /*BEF*/ + size_t(kRing) * 4 +
-           2 * list_cap * 4 + 256 /*P*/;
+__host__ __device__ constexpr size_t lds_fixed_bytes(size_t list_cap, size_t ring_cap, int has_lioff) {
+    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + 256 * 12 * 2 /*BEF*/ + ring_cap * 4 +
+           list_cap * 4 * (has_lioff ? 2 : 1) + 256 /*P*/;
 }
-__host__ __device__ inline size_t lds_table_bytes(int octaves) { return 2 * size_t(octaves) * kMaxL * sizeof(uw_axis); }
+__host__ __device__ inline size_t lds_table_bytes(int octaves, int axis_stride) {
+    return 2 * size_t(octaves) * size_t(axis_stride) * sizeof(uw_axis);
+}
 
 // Emission of one active cell: owned-edge lerps into verts (and the adjoint tape), then the
 // cell's indices. vbase: chunk-local vertex id of the cell's first owned record; ibase: global
@@ -244,12 +249,13 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     uint32_t* LUT = reinterpret_cast<uint32_t*>(TRI + 256);
     uint16_t* BEF = reinterpret_cast<uint16_t*>(LUT + 256);  // first-seen rank masks [256][12]
     uint32_t* RING = reinterpret_cast<uint32_t*>(BEF + 256 * 12);
-    uint32_t* LIST = RING + kRing;
-    uint32_t* LIOFF = LIST + a.list_cap;
-    uint8_t* P = reinterpret_cast<uint8_t*>(LIOFF + a.list_cap);  // list_cap is a multiple of 4
+    uint32_t* LIST = RING + a.ring_cap;
+    uint32_t* LIOFF = LIST + a.list_cap;  // x-slab chunks only (has_lioff)
+    // list_cap and ring_cap are multiples of 4, so the tables below stay 16-byte aligned
+    uint8_t* P = reinterpret_cast<uint8_t*>(LIST + a.list_cap * (a.has_lioff ? 2 : 1));
     uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);
-    uw_axis* AY = AX + O * kMaxL;
-    float* DENS = reinterpret_cast<float*>(AY + O * kMaxL);
+    uw_axis* AY = AX + O * a.axis_stride;
+    float* DENS = reinterpret_cast<float*>(AY + O * a.axis_stride);
     __shared__ unsigned long long scan_lds[W];
     __shared__ uint32_t wtot[W][3];
     __shared__ uint32_t s_task, s_bits;
@@ -268,11 +274,15 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     for (int i = tid; i < 256 * 12; i += BLOCK) BEF[i] = (&tab->before[0][0])[i];
     const float iso = a.f.iso;
 
+    // Thread 0 claims the next task one iteration ahead, so the queue atomic's round trip overlaps
+    // the current chunk's work (a claim past the end just ends the loop).
+    uint32_t claimed = tid == 0 ? atomicAdd(a.head, 1u) : 0u;
     for (;;) {
         __syncthreads();  // every wave is done with the previous task's shared state
         if (tid == 0) {
-            s_task = atomicAdd(a.head, 1u);
+            s_task = claimed;
             s_bits = 0u;
+            if (claimed < uint32_t(a.nchunks)) claimed = atomicAdd(a.head, 1u);
         }
         __syncthreads();
         const uint32_t task = s_task;
@@ -340,9 +350,9 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 if (g < c1) {
                     RING[g] = ((cv + (ex & 0x3FFu)) << 8) | cc.cs;
                     if (cc.act) {
+                        // cell (< 2048) | wave-local triangle prefix << 11
                         const uint32_t slot = c0 + ca + (ex >> 19);
-                        LIST[slot] = g;
-                        LIOFF[slot] = 3u * (ct + ((ex >> 10) & 0x1FFu));
+                        LIST[slot] = g | ((ct + ((ex >> 10) & 0x1FFu)) << 11);
                     }
                 }
                 const uint32_t tot = uw_wave_last(inc);
@@ -412,14 +422,15 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                         wa = abase[k];
                     }
                 const uint32_t slot = w * cpw + (e - wa);
-                const uint32_t g = LIST[slot];
+                const uint32_t ent = LIST[slot];
+                const uint32_t g = ent & 0x7FFu;
                 const uint32_t rw = RING[g];
                 int32_t x, y, z;
                 uw_cell_xyz(g, dn, dnn, &x, &y, &z);
                 float r[8];
                 cell_corners(DENS, dns, L, x, y, z, r);
-                emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + LIOFF[slot], step, ox, oy,
-                          oz, TRI, BEF, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
+                emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + 3u * (ent >> 11), step, ox,
+                          oy, oz, TRI, BEF, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
             }
             continue;
         }
@@ -554,6 +565,14 @@ BatchedKernel pick_kernel(int block, int octaves) {
     }
 }
 
+// True when every resolution up to max_res is built in one pass (one slab, n^3 <= list_cap):
+// the launch then needs no x-slab buffers (a kRing cell ring and the separate offset list).
+bool one_pass_only(const uw_batch_tuning& t, int32_t list_cap) {
+    for (int n = 1; n <= t.max_res; ++n)
+        if (t.slab[n] != n || n * n * n > list_cap) return false;
+    return true;
+}
+
 // Compacted-list capacity: every chunk built in one pass (one slab, n^3 <= kRing cells) must fit,
 // and a round of `block` cells must fit on top of a flushable list.
 int32_t list_cap_for(const uw_batch_tuning& t) {
@@ -630,7 +649,10 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         ctx->batch_ztab_bytes = bl->ztab_bytes;
     }
     bl->k = pick_kernel(tune.block, octaves);
-    bl->lds = lds_fixed_bytes(size_t(list_cap_for(tune))) + lds_table_bytes(octaves) + dens_bytes_for(tune);
+    const int32_t list_cap = list_cap_for(tune);
+    const bool one_pass = one_pass_only(tune, list_cap);
+    bl->lds = lds_fixed_bytes(size_t(list_cap), size_t(one_pass ? list_cap : kRing), one_pass ? 0 : 1) +
+              lds_table_bytes(octaves, tune.max_res + 1) + dens_bytes_for(tune);
     UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(bl->k), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(bl->lds)));
     bl->grid = tune.grid;
@@ -665,6 +687,10 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.eo = uw_make_emit_opts(out);
     a.max_res = tune.max_res;
     a.list_cap = list_cap_for(tune);
+    const bool one_pass = one_pass_only(tune, a.list_cap);
+    a.ring_cap = one_pass ? a.list_cap : kRing;
+    a.has_lioff = one_pass ? 0 : 1;
+    a.axis_stride = tune.max_res + 1;
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
     a.order = order;
     a.head = head;
