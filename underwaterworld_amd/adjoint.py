@@ -23,7 +23,9 @@ from .engine import MeshBatch, Perlin, _alloc_out, _coords_tensor, _mesh_out, _r
 class ChunkMesher:
     """Forward/backward state for one chunk set on one Perlin context."""
 
-    def __init__(self, perlin: Perlin, coords, res, want_dens_grad: bool = False):
+    def __init__(self, perlin: Perlin, coords, res, want_dens_grad: bool = False, exact_capacity: bool = False):
+        """exact_capacity: size the output (and tape) buffers from a counting launch instead of the
+        worst case 5 n (n+1)^2 vertices / 15 n^3 indices per chunk (large volumes)."""
         import torch
         self.perlin = perlin
         dev = torch.device("cuda", perlin.device)
@@ -32,6 +34,8 @@ class ChunkMesher:
         self.res = _res_tensor(res, self.nchunks, dev)
         rnp = self.res[: self.nchunks].cpu().numpy()
         self.vert_cap, self.ind_cap = batch_capacity(rnp)
+        if exact_capacity and self.nchunks:
+            self.vert_cap, self.ind_cap = self._required_sizes()
         self.max_samples = int(((rnp + 1) ** 3).max()) if self.nchunks else 1
         self.want_dens_grad = want_dens_grad
         self.batch: MeshBatch | None = None
@@ -39,6 +43,21 @@ class ChunkMesher:
         self.grad_amp = None   # (nchunks, octaves) float64
         self.grad_off = None   # (nchunks, 3) float64
         self.grad_dens = None  # (nchunks, max_samples) float32
+
+    def _required_sizes(self):
+        """Vertex and index counts of the set: a build with zero capacity overflows every chunk and
+        reports the required totals without writing a mesh."""
+        import torch
+        dev = self.coords.device
+        verts, inds, info, totals = _alloc_out(self.nchunks, 0, 0, dev)
+        out = _mesh_out(verts, inds, info, totals, 0, 0)
+        rc = _lib.lib().uw_build_chunks_batched(self.perlin.ctx, ctypes.c_void_p(self.coords.data_ptr()),
+                                                ctypes.c_void_p(self.res.data_ptr()), self.nchunks, ctypes.byref(out),
+                                                None)
+        _lib.check(rc, self.perlin.ctx)
+        torch.cuda.synchronize()
+        t = MeshBatch(verts, inds, info, totals, None).totals()
+        return int(t["n_verts"]), int(t["n_inds"])
 
     def set_field(self, amplitudes=None, offset=None):
         a = None if amplitudes is None else np.ascontiguousarray(amplitudes, np.float64)

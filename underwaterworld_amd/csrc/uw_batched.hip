@@ -164,47 +164,48 @@ template <class OwnerVoff>
 __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t L, int32_t x, int32_t y, int32_t z,
                                           uint32_t cs, const float r[8], uint32_t myv, uint64_t vo, uint64_t ib,
                                           float step, float ox, float oy, float oz, const uint64_t* TRI,
-                                          const uint16_t* BEF, const uint32_t* RING, uint32_t ring_mask,
-                                          OwnerVoff owner_voff, uint64_t vlim, uint64_t ilim) {
+                                          const uint32_t* LUT, const uint16_t* BEF, const uint32_t* RING,
+                                          uint32_t ring_mask, OwnerVoff owner_voff, uint64_t vlim, uint64_t ilim) {
     const float iso = a.f.iso;
     const uint32_t owned = uw_owned_mask(x, y, z);
-    const uint64_t row = TRI[cs];
-    const int nidx = 3 * int(row & 15u);
     const uint64_t vb = vo + myv;
     const uw_emit_opts& eo = a.eo;
+    // Vertices: one per owned edge of the case, at its rank in first-appearance order. Each lane
+    // loops over its own vertices only (the index loop below does no lerps), so the lerps do not
+    // run at the pace of the cell with the most index entries. vlim / ilim end the chunk's
+    // reserved ranges: writes stay inside them whatever the ids.
+    for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
+        const uint32_t ed = uint32_t(__builtin_ctz(vm));
+        const uint32_t rank = __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
+        if (vb + rank >= vlim) continue;
+        float p[3];
+        uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
+        float* dst = a.verts + eo.vstride * (vb + rank);
+        dst[0] = p[0];
+        dst[1] = p[1];
+        dst[2] = p[2];
+        if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
+        if (a.tape) {
+            const uint32_t ia = uw_edge_a(ed), ib2 = uw_edge_b(ed);
+            const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib2);
+            uw_vertex_tape tv;
+            tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L + z + int32_t((ca >> 2) & 1u));
+            tv.b = uint32_t(((x + int32_t(cb & 1u)) * L + y + int32_t((cb >> 1) & 1u)) * L + z + int32_t((cb >> 2) & 1u));
+            tv.rho_a = r[ia];
+            tv.rho_b = r[ib2];
+            a.tape[vb + rank] = tv;
+        }
+    }
+    // Indices, in the row's order; ids of records owned elsewhere come from the owner's ring entry.
+    const uint64_t row = TRI[cs];
+    const int nidx = 3 * int(row & 15u);
     const bool coll = eo.tri_normals || eo.tri_cells;
     float q0[3] = {0.f, 0.f, 0.f}, q1[3] = {0.f, 0.f, 0.f};
-    uint32_t seen = 0;
     for (int j = 0; j < nidx; ++j) {
         const uint32_t ed = uint32_t(row >> (4 + 4 * j)) & 15u;
-        const uint32_t bit = 1u << ed;
-        const bool fresh = (owned & bit) && !(seen & bit);
-        float p[3];
-        if (fresh || coll) uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
         uint32_t id;
-        if (owned & bit) {
-            const uint32_t rank = __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
-            id = myv + rank;
-            // vlim / ilim end the chunk's reserved ranges: writes stay inside them whatever the ids
-            if (fresh && vb + rank < vlim) {
-                float* dst = a.verts + eo.vstride * (vb + rank);
-                dst[0] = p[0];
-                dst[1] = p[1];
-                dst[2] = p[2];
-                if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
-                if (a.tape) {
-                    const uint32_t ia = uw_edge_a(ed), ib2 = uw_edge_b(ed);
-                    const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib2);
-                    uw_vertex_tape tv;
-                    tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L + z +
-                                    int32_t((ca >> 2) & 1u));
-                    tv.b = uint32_t(((x + int32_t(cb & 1u)) * L + y + int32_t((cb >> 1) & 1u)) * L + z +
-                                    int32_t((cb >> 2) & 1u));
-                    tv.rho_a = r[ia];
-                    tv.rho_b = r[ib2];
-                    a.tape[vb + rank] = tv;
-                }
-            }
+        if (owned & (1u << ed)) {
+            id = myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
         } else {
             int32_t qx, qy, qz;
             uint32_t qe;
@@ -213,10 +214,11 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
             const uint32_t qw = RING[q & ring_mask];
             id = owner_voff(q, qw) + __popc(uint32_t(BEF[(qw & 255u) * 12 + qe]) & uw_owned_mask(qx, qy, qz));
         }
-        seen |= bit;
         if (ib + j >= ilim) continue;
         uw_store_index(a.inds, eo.ind16, ib + j, id);
         if (coll) {
+            float p[3];
+            uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
             const int k = j % 3;
             if (k == 0) {
 #pragma unroll
@@ -274,15 +276,20 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     for (int i = tid; i < 256 * 12; i += BLOCK) BEF[i] = (&tab->before[0][0])[i];
     const float iso = a.f.iso;
 
-    // Thread 0 claims the next task one iteration ahead, so the queue atomic's round trip overlaps
-    // the current chunk's work (a claim past the end just ends the loop).
-    uint32_t claimed = tid == 0 ? atomicAdd(a.head, 1u) : 0u;
+    // While plenty of tasks remain, thread 0 claims the next task one iteration ahead, so the
+    // queue atomic's round trip overlaps the current chunk's work; near the end it claims at the
+    // loop top, so that no workgroup sits on a claimed task while finishing another.
+    const uint32_t lookahead_end = uint32_t(a.nchunks) > 2u * gridDim.x ? uint32_t(a.nchunks) - 2u * gridDim.x : 0u;
+    uint32_t claimed = 0u;
+    bool have_claim = false;
     for (;;) {
         __syncthreads();  // every wave is done with the previous task's shared state
         if (tid == 0) {
+            if (!have_claim) claimed = atomicAdd(a.head, 1u);
             s_task = claimed;
             s_bits = 0u;
-            if (claimed < uint32_t(a.nchunks)) claimed = atomicAdd(a.head, 1u);
+            have_claim = claimed < lookahead_end;
+            if (have_claim) claimed = atomicAdd(a.head, 1u);
         }
         __syncthreads();
         const uint32_t task = s_task;
@@ -430,7 +437,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 float r[8];
                 cell_corners(DENS, dns, L, x, y, z, r);
                 emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + 3u * (ent >> 11), step, ox,
-                          oy, oz, TRI, BEF, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
+                          oy, oz, TRI, LUT, BEF, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
             }
             continue;
         }
@@ -532,7 +539,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                         float r[8];
                         cell_corners(DENS, dns, L, x, y, z, r);
                         emit_cell(a, n, L, x, y, z, w & 255u, r, w >> 8, vo, io + LIOFF[e], step, ox, oy, oz, TRI,
-                                  BEF, RING, uint32_t(kRing - 1), owner_voff, vo + nv, io + ni);
+                                  LUT, BEF, RING, uint32_t(kRing - 1), owner_voff, vo + nv, io + ni);
                     }
                     __syncthreads();
                     list_n = 0;

@@ -226,7 +226,11 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                 oz = float(extent * coords[3 * chunk + 2]);
     __shared__ uint64_t s_tri[256];
     __shared__ uint16_t s_bef[256 * 12];
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) s_tri[i] = tab->tri[i];
+    __shared__ uint16_t s_used[256];
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+        s_tri[i] = tab->tri[i];
+        s_used[i] = tab->used[i];
+    }
     for (int i = threadIdx.x; i < 256 * 12; i += blockDim.x) s_bef[i] = (&tab->before[0][0])[i];
     __syncthreads();
     for (uint32_t a = threadIdx.x; a < ci.n_active; a += blockDim.x) {
@@ -242,33 +246,34 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
             r[i] = D[(x + int32_t(o & 1u)) * L * L + (y + int32_t((o >> 1) & 1u)) * L + z + int32_t((o >> 2) & 1u)];
         }
         const uint32_t owned = uw_owned_mask(x, y, z);
-        const uint64_t row = s_tri[cs];
-        const int nidx = 3 * int(row & 15u);
         const uint32_t myv = voff[cbase + cell];
         const uint64_t vb = ci.vert_off + myv;
         const uint64_t ib = ci.ind_off + ioff[cbase + a];
         // ends of the chunk's output ranges: writes stay inside them whatever the ids
         const uint64_t vlim = ci.vert_off + ci.n_verts, ilim = ci.ind_off + ci.n_inds;
+        // vertices: one per owned edge of the case, at its rank in first-appearance order
+        for (uint32_t vm = uint32_t(s_used[cs]) & owned; vm; vm &= vm - 1u) {
+            const uint32_t e = uint32_t(__builtin_ctz(vm));
+            const uint32_t rank = __popc(uint32_t(s_bef[cs * 12 + e]) & owned);
+            if (vb + rank >= vlim) continue;
+            float p[3];
+            uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
+            float* dst = verts + eo.vstride * (vb + rank);
+            dst[0] = p[0];
+            dst[1] = p[1];
+            dst[2] = p[2];
+            if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(e), float(extent), dst + 3);
+        }
+        // indices in the row's order; records owned elsewhere take the owner's id
+        const uint64_t row = s_tri[cs];
+        const int nidx = 3 * int(row & 15u);
         const bool coll = eo.tri_normals || eo.tri_cells;
         float q0[3] = {0.f, 0.f, 0.f}, q1[3] = {0.f, 0.f, 0.f};
-        uint32_t seen = 0;
         for (int j = 0; j < nidx; ++j) {
             const uint32_t e = uint32_t(row >> (4 + 4 * j)) & 15u;
-            const uint32_t bit = 1u << e;
-            const bool fresh = (owned & bit) && !(seen & bit);
-            float p[3];
-            if (fresh || coll) uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
             uint32_t id;
-            if (owned & bit) {
-                const uint32_t rank = __popc(uint32_t(s_bef[cs * 12 + e]) & owned);
-                id = myv + rank;
-                if (fresh && vb + rank < vlim) {
-                    float* dst = verts + eo.vstride * (vb + rank);
-                    dst[0] = p[0];
-                    dst[1] = p[1];
-                    dst[2] = p[2];
-                    if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(e), float(extent), dst + 3);
-                }
+            if (owned & (1u << e)) {
+                id = myv + __popc(uint32_t(s_bef[cs * 12 + e]) & owned);
             } else {
                 int32_t qx, qy, qz;
                 uint32_t qe;
@@ -277,10 +282,11 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
                 const uint32_t qc = cases[cbase + q];
                 id = voff[cbase + q] + __popc(uint32_t(s_bef[qc * 12 + qe]) & uw_owned_mask(qx, qy, qz));
             }
-            seen |= bit;
             if (ib + j >= ilim) continue;
             uw_store_index(inds, eo.ind16, ib + j, id);
             if (coll) {
+                float p[3];
+                uw_edge_vertex(e, x, y, z, r, iso, step, ox, oy, oz, p);
                 const int k = j % 3;
                 if (k == 0) {
 #pragma unroll
