@@ -276,20 +276,11 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     for (int i = tid; i < 256 * 12; i += BLOCK) BEF[i] = (&tab->before[0][0])[i];
     const float iso = a.f.iso;
 
-    // While plenty of tasks remain, thread 0 claims the next task one iteration ahead, so the
-    // queue atomic's round trip overlaps the current chunk's work; near the end it claims at the
-    // loop top, so that no workgroup sits on a claimed task while finishing another.
-    const uint32_t lookahead_end = uint32_t(a.nchunks) > 2u * gridDim.x ? uint32_t(a.nchunks) - 2u * gridDim.x : 0u;
-    uint32_t claimed = 0u;
-    bool have_claim = false;
     for (;;) {
         __syncthreads();  // every wave is done with the previous task's shared state
         if (tid == 0) {
-            if (!have_claim) claimed = atomicAdd(a.head, 1u);
-            s_task = claimed;
+            s_task = atomicAdd(a.head, 1u);
             s_bits = 0u;
-            have_claim = claimed < lookahead_end;
-            if (have_claim) claimed = atomicAdd(a.head, 1u);
         }
         __syncthreads();
         const uint32_t task = s_task;
