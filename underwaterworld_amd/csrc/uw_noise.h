@@ -199,15 +199,17 @@ __device__ __forceinline__ void uw_build_chunk_tables(const uw_field& f, const u
 __device__ __forceinline__ void uw_build_xy_tables(const uw_field& f, const uint8_t* perm, int32_t cx, int32_t cy,
                                                    int32_t L, float step, uw_axis* AX, uw_axis* AY, int tid,
                                                    int nthr) {
-    const int32_t O = f.octaves;
-    const uw_divider dL = uw_make_divider(uint32_t(L));
-    for (int e = tid; e < 2 * O * L; e += nthr) {
-        const bool is_y = e >= O * L;
-        const int32_t r = is_y ? e - O * L : e;
-        const int32_t k = int32_t(uw_div(uint32_t(r), dL)), i = r - k * L;
-        const double fk = f.octave_freq[k];
-        if (is_y) AY[r] = uw_axis_of(uw_sample_coord(f, 1, cy, i, step), fk);
-        else AX[r] = uw_hash_x(uw_axis_of(uw_sample_coord(f, 0, cx, i, step), fk), perm);
+    // one thread per (axis, sample); the octave loop is uniform, so the octave frequencies are
+    // read as scalars and the sample coordinate is formed once
+    for (int e = tid; e < 2 * L; e += nthr) {
+        const bool is_y = e >= L;
+        const int32_t i = is_y ? e - L : e;
+        const double p = uw_sample_coord(f, is_y ? 1 : 0, is_y ? cy : cx, i, step);
+        for (int32_t k = 0; k < f.octaves; ++k) {
+            const uw_axis v = uw_axis_of(p, f.octave_freq[k]);
+            if (is_y) AY[k * L + i] = v;
+            else AX[k * L + i] = uw_hash_x(v, perm);
+        }
     }
 }
 
@@ -255,13 +257,10 @@ __device__ __forceinline__ uw_axis uw_sload(const uw_axis* p) {
 // One chunk's z table (chunk coordinate cz, L samples per axis).
 __device__ __forceinline__ void uw_fill_ztab(const uw_field& f, int32_t cz, int32_t L, float step, uw_axis* az,
                                              float* adj, float* post, int32_t stride, int tid, int nthr) {
-    const int32_t O = f.octaves;
-    const uw_divider dL = uw_make_divider(uint32_t(L));
-    for (int e = tid; e < (O + 1) * L; e += nthr) {
-        const int32_t k = int32_t(uw_div(uint32_t(e), dL)), i = e - k * L;
+    for (int i = tid; i < L; i += nthr) {
         const double pz = uw_sample_coord(f, 2, cz, i, step);
-        if (k < O) az[k * stride + i] = uw_axis_of(pz, f.octave_freq[k]);
-        else uw_bias_terms(f, pz, &adj[i], &post[i]);
+        for (int32_t k = 0; k < f.octaves; ++k) az[k * stride + i] = uw_axis_of(pz, f.octave_freq[k]);
+        uw_bias_terms(f, pz, &adj[i], &post[i]);
     }
 }
 
