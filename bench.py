@@ -196,7 +196,8 @@ def main():
 
         def adj_step():
             mesher.forward(stream.cuda_stream)
-            mesher.grad_dens.zero_() if mesher.grad_dens is not None else None
+            if mesher.grad_dens is not None:
+                mesher.grad_dens.zero_()
             mesher.backward(gv, stream.cuda_stream)
 
         ms_adj = timed(adj_step, max(3, args.steps // 4), 1)
@@ -235,7 +236,8 @@ def main():
 
         def vol_step():
             vm.forward(stream.cuda_stream)
-            vm.grad_dens.zero_()
+            if vm.grad_dens is not None:
+                vm.grad_dens.zero_()
             vm.backward(gvv, stream.cuda_stream)
 
         ms_vadj = timed(vol_step, 2, 1)
