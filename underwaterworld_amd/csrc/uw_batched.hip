@@ -160,26 +160,19 @@ __host__ __device__ inline size_t lds_table_bytes(int octaves, int axis_stride) 
 // cell's indices. vbase: chunk-local vertex id of the cell's first owned record; ibase: global
 // index of its first index; owner_voff(q) gives the chunk-local vertex base of cell q; RING holds
 // (local base << 8 | case) per cell.
-template <class OwnerVoff>
-__device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t L, int32_t x, int32_t y, int32_t z,
-                                          uint32_t cs, const float r[8], uint32_t myv, uint64_t vo, uint64_t ib,
-                                          float step, float ox, float oy, float oz, const uint64_t* TRI,
-                                          const uint32_t* LUT, const uint16_t* BEF, const uint32_t* RING,
-                                          uint32_t ring_mask, OwnerVoff owner_voff, uint64_t vlim, uint64_t ilim) {
-    const float iso = a.f.iso;
-    const uint32_t owned = uw_owned_mask(x, y, z);
-    const uint64_t vb = vo + myv;
+// Vertices of one active cell: one per owned edge of the case, at its rank in first-appearance
+// order. vlim ends the chunk's reserved vertex range: writes stay inside it whatever the ids.
+__device__ __forceinline__ void emit_vertices(const BatchArgs& a, int32_t L, int32_t x, int32_t y, int32_t z,
+                                              uint32_t cs, uint32_t owned, const float r[8], uint64_t vb, float step,
+                                              float ox, float oy, float oz, const uint32_t* LUT, const uint16_t* BEF,
+                                              uint64_t vlim) {
     const uw_emit_opts& eo = a.eo;
-    // Vertices: one per owned edge of the case, at its rank in first-appearance order. Each lane
-    // loops over its own vertices only (the index loop below does no lerps), so the lerps do not
-    // run at the pace of the cell with the most index entries. vlim / ilim end the chunk's
-    // reserved ranges: writes stay inside them whatever the ids.
     for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
         const uint32_t ed = uint32_t(__builtin_ctz(vm));
         const uint32_t rank = __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
         if (vb + rank >= vlim) continue;
         float p[3];
-        uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
+        uw_edge_vertex(ed, x, y, z, r, a.f.iso, step, ox, oy, oz, p);
         float* dst = a.verts + eo.vstride * (vb + rank);
         dst[0] = p[0];
         dst[1] = p[1];
@@ -196,29 +189,48 @@ __device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t
             a.tape[vb + rank] = tv;
         }
     }
-    // Indices, in the row's order; ids of records owned elsewhere come from the owner's ring entry.
+}
+
+// Chunk-local vertex id of the record cell (x, y, z) sees as edge ed: its own rank when it owns
+// the edge, else the owner's (owner_voff(q) gives the vertex base of cell q; RING holds
+// (local base << 8 | case) per cell).
+template <class OwnerVoff>
+__device__ __forceinline__ uint32_t edge_id(int32_t n, int32_t x, int32_t y, int32_t z, uint32_t cs, uint32_t owned,
+                                            uint32_t ed, uint32_t myv, const uint16_t* BEF, const uint32_t* RING,
+                                            uint32_t ring_mask, OwnerVoff owner_voff) {
+    if (owned & (1u << ed)) return myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
+    int32_t qx, qy, qz;
+    uint32_t qe;
+    uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
+    const uint32_t q = uint32_t((qx * n + qy) * n + qz);
+    const uint32_t qw = RING[q & ring_mask];
+    return owner_voff(q, qw) + __popc(uint32_t(BEF[(qw & 255u) * 12 + qe]) & uw_owned_mask(qx, qy, qz));
+}
+
+// Emission of one active cell: its vertices (and the adjoint tape), then its index entries in the
+// row's order, and, when requested, its collision records. ib: global index of its first index;
+// vlim / ilim end the chunk's reserved ranges.
+template <class OwnerVoff>
+__device__ __forceinline__ void emit_cell(const BatchArgs& a, int32_t n, int32_t L, int32_t x, int32_t y, int32_t z,
+                                          uint32_t cs, const float r[8], uint32_t myv, uint64_t vo, uint64_t ib,
+                                          float step, float ox, float oy, float oz, const uint64_t* TRI,
+                                          const uint32_t* LUT, const uint16_t* BEF, const uint32_t* RING,
+                                          uint32_t ring_mask, OwnerVoff owner_voff, uint64_t vlim, uint64_t ilim) {
+    const uint32_t owned = uw_owned_mask(x, y, z);
+    const uw_emit_opts& eo = a.eo;
+    emit_vertices(a, L, x, y, z, cs, owned, r, vo + myv, step, ox, oy, oz, LUT, BEF, vlim);
     const uint64_t row = TRI[cs];
     const int nidx = 3 * int(row & 15u);
     const bool coll = eo.tri_normals || eo.tri_cells;
     float q0[3] = {0.f, 0.f, 0.f}, q1[3] = {0.f, 0.f, 0.f};
     for (int j = 0; j < nidx; ++j) {
         const uint32_t ed = uint32_t(row >> (4 + 4 * j)) & 15u;
-        uint32_t id;
-        if (owned & (1u << ed)) {
-            id = myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
-        } else {
-            int32_t qx, qy, qz;
-            uint32_t qe;
-            uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
-            const uint32_t q = uint32_t((qx * n + qy) * n + qz);
-            const uint32_t qw = RING[q & ring_mask];
-            id = owner_voff(q, qw) + __popc(uint32_t(BEF[(qw & 255u) * 12 + qe]) & uw_owned_mask(qx, qy, qz));
-        }
+        const uint32_t id = edge_id(n, x, y, z, cs, owned, ed, myv, BEF, RING, ring_mask, owner_voff);
         if (ib + j >= ilim) continue;
         uw_store_index(a.inds, eo.ind16, ib + j, id);
         if (coll) {
             float p[3];
-            uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
+            uw_edge_vertex(ed, x, y, z, r, a.f.iso, step, ox, oy, oz, p);
             const int k = j % 3;
             if (k == 0) {
 #pragma unroll
@@ -407,6 +419,11 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 for (int w = 0; w < W; ++w) b = (uint32_t(w) == wq) ? vbase[w] : b;
                 return b + (qw >> 8);
             };
+            // Triangle-parallel indices: when the chunk's triangle -> list-slot map fits the density
+            // slab (which the index pass no longer reads), the per-cell loop writes only vertices and
+            // every thread then takes whole triangles, so no lane waits for the cell with the most
+            // index entries. Collision records need the densities and keep the per-cell loop.
+            const bool flat = !(a.eo.tri_normals || a.eo.tri_cells) && nt <= uint32_t(ns * L * L * 2);
             for (uint32_t e = tid; e < na; e += BLOCK) {
                 uint32_t w = 0;
 #pragma unroll
@@ -427,8 +444,61 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 uw_cell_xyz(g, dn, dnn, &x, &y, &z);
                 float r[8];
                 cell_corners(DENS, dns, L, x, y, z, r);
-                emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + 3u * (ent >> 11), step, ox,
-                          oy, oz, TRI, LUT, BEF, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
+                if (flat)
+                    emit_vertices(a, L, x, y, z, rw & 255u, uw_owned_mask(x, y, z), r, vo + wb + (rw >> 8), step, ox,
+                                  oy, oz, LUT, BEF, vo + nv);
+                else
+                    emit_cell(a, n, L, x, y, z, rw & 255u, r, wb + (rw >> 8), vo, io + wi + 3u * (ent >> 11), step,
+                              ox, oy, oz, TRI, LUT, BEF, RING, 0xFFFFFFFFu, owner_voff, vo + nv, io + ni);
+            }
+            if (flat) {
+                __syncthreads();  // the density slab is read no more in this chunk
+                uint16_t* T2C = reinterpret_cast<uint16_t*>(DENS);
+                for (uint32_t e = tid; e < na; e += BLOCK) {
+                    uint32_t w = 0;
+#pragma unroll
+                    for (int k = 1; k < W; ++k) w += (e >= abase[k]) ? 1u : 0u;
+                    uint32_t wi = 0, wa = 0;
+#pragma unroll
+                    for (int k = 0; k < W; ++k)
+                        if (uint32_t(k) == w) {
+                            wi = ibase[k];
+                            wa = abase[k];
+                        }
+                    const uint32_t slot = w * cpw + (e - wa);
+                    const uint32_t ent = LIST[slot];
+                    const uint32_t tp = wi / 3u + (ent >> 11);
+                    const uint32_t ntri = uint32_t(TRI[RING[ent & 0x7FFu] & 255u] & 15u);
+                    for (uint32_t i = 0; i < ntri; ++i) T2C[tp + i] = uint16_t(slot);
+                }
+                __syncthreads();
+                for (uint32_t t = tid; t < nt; t += BLOCK) {
+                    const uint32_t slot = T2C[t];
+                    const uint32_t w = uw_div(slot, dcpw);
+                    uint32_t wb = 0, wi = 0;
+#pragma unroll
+                    for (int k = 0; k < W; ++k)
+                        if (uint32_t(k) == w) {
+                            wb = vbase[k];
+                            wi = ibase[k];
+                        }
+                    const uint32_t ent = LIST[slot];
+                    const uint32_t g = ent & 0x7FFu;
+                    const uint32_t rw = RING[g];
+                    const uint32_t cs = rw & 255u, myv = wb + (rw >> 8);
+                    const uint32_t j0 = 3u * (t - (wi / 3u + (ent >> 11)));
+                    int32_t x, y, z;
+                    uw_cell_xyz(g, dn, dnn, &x, &y, &z);
+                    const uint32_t owned = uw_owned_mask(x, y, z);
+                    const uint64_t row = TRI[cs];
+#pragma unroll
+                    for (uint32_t k = 0; k < 3; ++k) {
+                        const uint32_t ed = uint32_t(row >> (4 + 4 * (j0 + k))) & 15u;
+                        const uint32_t id = edge_id(n, x, y, z, cs, owned, ed, myv, BEF, RING, 0xFFFFFFFFu, owner_voff);
+                        const uint64_t gi = io + 3ull * t + k;
+                        if (gi < io + ni) uw_store_index(a.inds, a.eo.ind16, gi, id);
+                    }
+                }
             }
             continue;
         }

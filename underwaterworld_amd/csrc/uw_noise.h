@@ -114,14 +114,20 @@ __device__ __forceinline__ float uw_density_from(float total, float amp_sum, flo
 //   g_ijl(dz) = Gx dx_i + Gy dy_j + Gz (dz - l) = P_ijl + Gz_ijl dz_l.
 // The x and y lerps are linear, so the two z faces of the cell reduce to
 //   N_l(dz) = A_l + B_l dz_l,  A_l = lerp_y(lerp_x(P_.. l)),  B_l = lerp_y(lerp_x(Gz_.. l)),
-// and a sample costs N0, N1 (two FMAs), one lerp by the z fade and the octave accumulation. The
-// A/B setup (eight gradient fetches and ~40 FLOPs) is redone only when the line enters a new
-// lattice cell of that octave, which is wave-uniform because every lane of a wave walks its own
-// line at the same z index.
+// and a sample is N0 + s(dz) (N1 - N0) with N1 - N0 = C + D dz: three FMAs, the clamp and the
+// octave accumulation. The A/B setup (gradient fetches and ~20 FLOPs per face) is redone only when
+// the line enters a new lattice cell of that octave, which is wave-uniform because every lane of a
+// wave walks its own line at the same z index.
 
 struct uw_line_oct {
     float A0, B0, A1, B1;
+    float C, D;  // N1 - N0 = C + D dz with C = (A1 - B1) - A0, D = B1 - B0
 };
+
+__device__ __forceinline__ void uw_line_diff(uw_line_oct& s) {
+    s.C = (s.A1 - s.B1) - s.A0;
+    s.D = s.B1 - s.B0;
+}
 
 // One z face of a line's lattice cell: A = lerp_y(lerp_x(P_..)), B = lerp_y(lerp_x(Gz_..)) for the
 // face whose z lattice byte is zb. bb = b00 | b01 << 8 | b10 << 16 | b11 << 24 (second-level
@@ -143,6 +149,7 @@ __device__ __forceinline__ uw_line_oct uw_line_setup(const uw_axis& X, const uw_
     uw_line_oct s;
     uw_line_face(X, Y, bb, zbits & 255u, GP, &s.A0, &s.B0);
     uw_line_face(X, Y, bb, (zbits >> 8) & 255u, GP, &s.A1, &s.B1);
+    uw_line_diff(s);
     return s;
 }
 
@@ -155,6 +162,7 @@ __device__ __forceinline__ void uw_line_advance(uw_line_oct& s, uint32_t prev, c
         s.A0 = s.A1;
         s.B0 = s.B1;
         uw_line_face(X, Y, bb, (zbits >> 8) & 255u, GP, &s.A1, &s.B1);
+        uw_line_diff(s);
     } else {
         s = uw_line_setup(X, Y, bb, zbits, GP);
     }
@@ -162,10 +170,11 @@ __device__ __forceinline__ void uw_line_advance(uw_line_oct& s, uint32_t prev, c
 
 // Unscaled octave value at the line sample, clamped to the unscaled image of [-1, 1]; the caller
 // multiplies by the octave weight A_k (2/sqrt 3) / sum A.
+// N0 + s(dz) (N1 - N0), with the face difference N1 - N0 = C + D dz formed by one FMA.
 __device__ __forceinline__ float uw_line_sample(const uw_line_oct& s, const uw_axis& Z, float lo, float hi) {
     const float n0 = fmaf(s.B0, Z.d, s.A0);
-    const float n1 = fmaf(s.B1, Z.dm1, s.A1);
-    return __builtin_amdgcn_fmed3f(uw_lerp(Z.fade, n0, n1), lo, hi);
+    const float dn = fmaf(s.D, Z.d, s.C);
+    return __builtin_amdgcn_fmed3f(fmaf(Z.fade, dn, n0), lo, hi);
 }
 
 // Per-chunk LDS tables: every (axis index, octave) entry of the chunk and the z-bias split.
