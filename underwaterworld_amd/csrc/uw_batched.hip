@@ -286,6 +286,8 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         LUT[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
     }
     for (int i = tid; i < 256 * 12; i += BLOCK) BEF[i] = (&tab->before[0][0])[i];
+    __shared__ double s_freq[UW_MAX_OCTAVES];
+    if (tid < UW_MAX_OCTAVES) s_freq[tid] = a.f.octave_freq[tid];
     const float iso = a.f.iso;
 
     for (;;) {
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         const float step = float(a.f.extent) / float(n);
         const float ox = float(a.f.extent * cx), oy = float(a.f.extent * cy), oz = float(a.f.extent * cz);
         const uw_chunk_tables T = {AX, AY, nullptr, nullptr, nullptr};
-        uw_build_xy_tables(a.f, P, cx, cy, L, step, AX, AY, tid, BLOCK);
+        uw_build_xy_tables(a.f, s_freq, P, cx, cy, L, step, AX, AY, tid, BLOCK);
         const uw_ztab Zt = uw_ztab_of(a.ztab, c, O, a.zstride);
         __syncthreads();
 

@@ -205,20 +205,20 @@ __device__ __forceinline__ void uw_build_chunk_tables(const uw_field& f, const u
 
 // The x and y tables only (the density loops read the z entries from the chunk's z table in HBM,
 // see uw_ztab): AX[k L + i] hashed, AY[k L + i].
-__device__ __forceinline__ void uw_build_xy_tables(const uw_field& f, const uint8_t* perm, int32_t cx, int32_t cy,
-                                                   int32_t L, float step, uw_axis* AX, uw_axis* AY, int tid,
-                                                   int nthr) {
-    // one thread per (axis, sample); the octave loop is uniform, so the octave frequencies are
-    // read as scalars and the sample coordinate is formed once
-    for (int e = tid; e < 2 * L; e += nthr) {
-        const bool is_y = e >= L;
-        const int32_t i = is_y ? e - L : e;
-        const double p = uw_sample_coord(f, is_y ? 1 : 0, is_y ? cy : cx, i, step);
-        for (int32_t k = 0; k < f.octaves; ++k) {
-            const uw_axis v = uw_axis_of(p, f.octave_freq[k]);
-            if (is_y) AY[k * L + i] = v;
-            else AX[k * L + i] = uw_hash_x(v, perm);
-        }
+// One thread per (axis, octave, sample) entry, so the short f64 chains of a chunk's entries run
+// side by side; freq holds the octave frequencies (an LDS copy, indexed per lane).
+__device__ __forceinline__ void uw_build_xy_tables(const uw_field& f, const double* freq, const uint8_t* perm,
+                                                   int32_t cx, int32_t cy, int32_t L, float step, uw_axis* AX,
+                                                   uw_axis* AY, int tid, int nthr) {
+    const int32_t OL = f.octaves * L;
+    const uw_divider dL = uw_make_divider(uint32_t(L));
+    for (int e = tid; e < 2 * OL; e += nthr) {
+        const bool is_y = e >= OL;
+        const int32_t r = is_y ? e - OL : e;
+        const int32_t k = int32_t(uw_div(uint32_t(r), dL)), i = r - k * L;
+        const uw_axis v = uw_axis_of(uw_sample_coord(f, is_y ? 1 : 0, is_y ? cy : cx, i, step), freq[k]);
+        if (is_y) AY[r] = v;
+        else AX[r] = uw_hash_x(v, perm);
     }
 }
 

@@ -48,17 +48,18 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     float* SLAB = reinterpret_cast<float*>(AY + O * L);            // [planes][L][L]
 
     __shared__ uint32_t s_bits;
-    const int32_t cx = coords[3 * chunk], cy = coords[3 * chunk + 1], cz = coords[3 * chunk + 2];
+    __shared__ double s_freq[UW_MAX_OCTAVES];
+    const int32_t cx = coords[3 * chunk], cy = coords[3 * chunk + 1];
     if (threadIdx.x == 0) s_bits = 0u;
+    if (threadIdx.x < UW_MAX_OCTAVES) s_freq[threadIdx.x] = f.octave_freq[threadIdx.x];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
         GP[i] = tab->grad[p & 15];
     }
     __syncthreads();
-    (void)cz;
     const uw_chunk_tables T = {AX, AY, nullptr, nullptr, nullptr};
-    uw_build_xy_tables(f, P, cx, cy, L, step, AX, AY, threadIdx.x, blockDim.x);
+    uw_build_xy_tables(f, s_freq, P, cx, cy, L, step, AX, AY, threadIdx.x, blockDim.x);
     const uw_ztab Zt = uw_ztab_of(ztab, chunk, O, L);
     __syncthreads();
     // plane x lands in slot x % planes = x - x0
