@@ -26,17 +26,21 @@
 namespace {
 
 constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
-constexpr int kMaxL = kMaxN + 1;
 constexpr int kRing = 2048;       // cell ring (vertex base << 8 | case), power of two
 constexpr int kMinList = 512;     // smallest compacted active-cell list
 constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
 
+// One queued chunk, in queue order (32 bytes, two 16-byte loads): chunk index c (-1: queue
+// empty), resolution n and chunk coordinates. Written by k_order, read once per task.
+struct __align__(16) uw_task {
+    int32_t c, n, cx, cy;
+    int32_t cz, pad[3];
+};
+
 struct BatchArgs {
     uw_field f;
     const uw_tables* tab;
-    const int32_t* coords;
-    const int32_t* res;
-    const uint32_t* order;
+    const uw_task* tasks;
     int32_t nchunks;
     uint32_t* head;
     uw_chunk_info* info;
@@ -64,16 +68,48 @@ __global__ void k_reset(uint32_t* head, uw_totals* totals) {
     }
 }
 
-// Counting sort of chunk indices by resolution, largest first (one workgroup).
-__global__ void __launch_bounds__(1024) k_order(const int32_t* __restrict__ res, int32_t nchunks,
-                                                uint32_t* __restrict__ order) {
+// Adds the wave's lanes whose bucket equals b to cnt[b], one LDS atomic per distinct bucket in
+// the wave, and returns each valid lane's slot: the bucket's old count plus its rank among the
+// wave's lanes of that bucket.
+__device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* cnt, int b, bool valid) {
+    const int lane = threadIdx.x & 63;
+    uint64_t todo = __ballot(valid);
+    uint32_t slot = 0;
+    while (todo) {
+        const int lead = __ffsll((unsigned long long)todo) - 1;
+        const int bl = __shfl(b, lead);
+        const uint64_t m = __ballot(valid && b == bl);
+        uint32_t base = 0;
+        if (lane == lead) base = atomicAdd(&cnt[bl], uint32_t(__popcll(m)));
+        base = __shfl(base, lead);
+        if (valid && b == bl) slot = base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+        todo &= ~m;
+    }
+    return slot;
+}
+
+// Queue set-up (one workgroup): resets the output totals and the queue head (to first_task: the
+// kernel's workgroup g starts with task g without claiming it), and writes the task records in
+// queue order, a counting sort by resolution, largest first, invalid resolutions last.
+__global__ void __launch_bounds__(1024) k_order(const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
+                                                int32_t nchunks, uw_task* __restrict__ tasks, uint32_t* head,
+                                                uw_totals* totals, uint32_t first_task) {
     __shared__ uint32_t cnt[kMaxN + 2];
+    if (threadIdx.x == 0) {
+        *head = first_task;
+        uw_totals z = {0, 0, 0, 0};
+        *totals = z;
+    }
     for (int i = threadIdx.x; i < kMaxN + 2; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
-    for (int32_t i = threadIdx.x; i < nchunks; i += blockDim.x) {
-        const int32_t n = res[i];
-        const int b = (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1;  // descending n, invalid last
-        atomicAdd(&cnt[b], 1u);
+    // the loops run whole waves (i0 uniform per wave) so the bucket adds can use ballots
+    const int32_t stride = int32_t(blockDim.x);
+    for (int32_t i0 = 0; i0 < nchunks; i0 += stride) {
+        const int32_t i = i0 + int32_t(threadIdx.x);
+        const bool valid = i < nchunks;
+        const int32_t n = valid ? res[i] : 0;
+        const int b = (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1;
+        wave_bucket_add(cnt, b, valid);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -85,11 +121,40 @@ __global__ void __launch_bounds__(1024) k_order(const int32_t* __restrict__ res,
         }
     }
     __syncthreads();
-    for (int32_t i = threadIdx.x; i < nchunks; i += blockDim.x) {
-        const int32_t n = res[i];
+    for (int32_t i0 = 0; i0 < nchunks; i0 += stride) {
+        const int32_t i = i0 + int32_t(threadIdx.x);
+        const bool valid = i < nchunks;
+        const int32_t n = valid ? res[i] : 0;
         const int b = (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1;
-        order[atomicAdd(&cnt[b], 1u)] = uint32_t(i);
+        const uint32_t slot = wave_bucket_add(cnt, b, valid);
+        if (valid) {
+            uw_task t;
+            t.c = i;
+            t.n = n;
+            t.cx = coords[3 * i];
+            t.cy = coords[3 * i + 1];
+            t.cz = coords[3 * i + 2];
+            t.pad[0] = t.pad[1] = t.pad[2] = 0;
+            tasks[slot] = t;
+        }
     }
+}
+
+// Copies the record of queue task t (c = -1 past the end of the queue) and t to shared memory.
+__device__ __forceinline__ void stage_task(const BatchArgs& a, uint32_t t, int32_t* rec) {
+    int4 r0 = make_int4(-1, 0, 0, 0);
+    int32_t cz = 0;
+    if (t < uint32_t(a.nchunks)) {
+        const int4* p = reinterpret_cast<const int4*>(a.tasks + t);
+        r0 = p[0];
+        cz = p[1].x;
+    }
+    rec[0] = r0.x;
+    rec[1] = r0.y;
+    rec[2] = r0.z;
+    rec[3] = r0.w;
+    rec[4] = cz;
+    rec[5] = int32_t(t);
 }
 
 template <int BLOCK>
@@ -272,7 +337,8 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     float* DENS = reinterpret_cast<float*>(AY + O * a.axis_stride);
     __shared__ unsigned long long scan_lds[W];
     __shared__ uint32_t wtot[W][3];
-    __shared__ uint32_t s_task, s_bits;
+    __shared__ int32_t s_next[6];  // c, n, cx, cy, cz and queue index of the next task
+    __shared__ uint32_t s_bits;
     __shared__ unsigned long long s_vo, s_io;
     __shared__ int s_emit;
 
@@ -290,18 +356,35 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     if (tid < UW_MAX_OCTAVES) s_freq[tid] = a.f.octave_freq[tid];
     const float iso = a.f.iso;
 
+    // Task queue. Workgroup g starts with task g (the queue head starts at the grid size). While
+    // at least two grids' worth of tasks lie beyond the current one, a one-pass task is given one
+    // task of lookahead: the block's last lane, the one with the least density work, claims the
+    // next task and stages its record in shared memory once its own share of the density
+    // evaluation is done, so the two dependent memory round trips (queue head, then record)
+    // overlap the other lanes' work instead of opening the next task. Near the end of the queue,
+    // and for multi-slab tasks (long enough that the round trips hardly count), the next task is
+    // claimed when the block is free, which keeps the end of the launch balanced.
+    const bool pf = tid == BLOCK - 1;
+    if (pf) stage_task(a, blockIdx.x, s_next);
+    bool staged = true;  // block-uniform: s_next holds the next task
+
     for (;;) {
         __syncthreads();  // every wave is done with the previous task's shared state
-        if (tid == 0) {
-            s_task = atomicAdd(a.head, 1u);
-            s_bits = 0u;
+        if (!staged) {
+            if (pf) stage_task(a, atomicAdd(a.head, 1u), s_next);
+            __syncthreads();
         }
-        __syncthreads();
-        const uint32_t task = s_task;
-        if (task >= uint32_t(a.nchunks)) break;
         // chunk-level values are wave-uniform: keep them in scalar registers
-        const int32_t c = __builtin_amdgcn_readfirstlane(int32_t(a.order[task]));
-        const int32_t n = __builtin_amdgcn_readfirstlane(a.res[c]);
+        const int32_t c = __builtin_amdgcn_readfirstlane(s_next[0]);
+        if (c < 0) break;
+        const int32_t n = __builtin_amdgcn_readfirstlane(s_next[1]);
+        const int32_t cx = __builtin_amdgcn_readfirstlane(s_next[2]);
+        const int32_t cy = __builtin_amdgcn_readfirstlane(s_next[3]);
+        const int32_t cz = __builtin_amdgcn_readfirstlane(s_next[4]);
+        const bool lookahead = uint32_t(__builtin_amdgcn_readfirstlane(s_next[5])) + 2u * gridDim.x <
+                               uint32_t(a.nchunks);
+        staged = false;
+        if (tid == 0) s_bits = 0u;  // first set after the table barrier below
         if (n < 1 || n > a.max_res || !uw_octaves<OCT>::accepts(O)) {
             if (tid == 0) {
                 uw_chunk_info r = {UW_CHUNK_INVALID | UW_CHUNK_BLANK, 0, 0, 0, 0, 0};
@@ -311,9 +394,6 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         }
         const int32_t L = n + 1, nn = n * n, C = nn * n;
         const uw_divider dn = uw_make_divider(uint32_t(n)), dnn = uw_make_divider(uint32_t(nn));
-        const int32_t cx = __builtin_amdgcn_readfirstlane(a.coords[3 * c]);
-        const int32_t cy = __builtin_amdgcn_readfirstlane(a.coords[3 * c + 1]);
-        const int32_t cz = __builtin_amdgcn_readfirstlane(a.coords[3 * c + 2]);
         const float step = float(a.f.extent) / float(n);
         const float ox = float(a.f.extent * cx), oy = float(a.f.extent * cy), oz = float(a.f.extent * cz);
         const uw_chunk_tables T = {AX, AY, nullptr, nullptr, nullptr};
@@ -329,6 +409,11 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         if (nslab == 1 && C <= a.list_cap) {
             // ---------- one density evaluation, one classification pass ----------
             const int bits = uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, 0, L, DENS, ns, tid, BLOCK);
+            // s_next was read before the table barrier; the next read follows the end-of-task barrier
+            if (lookahead) {
+                if (pf) stage_task(a, atomicAdd(a.head, 1u), s_next);
+                staged = true;
+            }
             if (bits) atomicOr(&s_bits, uint32_t(bits));
             __syncthreads();
             const uint32_t sb = s_bits;
@@ -512,6 +597,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
             below |= uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
                      UW_BELOW_NOT_ABOVE;
+
             __syncthreads();
             for (int32_t g = xa * nn + tid; g < xe * nn; g += BLOCK) {
                 int32_t x, y, z;
@@ -706,7 +792,7 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         (void)hipFree(ctx->d_batch_order);
         ctx->d_batch_order = nullptr;
         ctx->batch_order_cap = 0;
-        UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, size_t(nchunks) * sizeof(uint32_t)));
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, size_t(nchunks) * sizeof(uw_task)));
         ctx->batch_order_cap = size_t(nchunks);
     }
     if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, sizeof(uint32_t)));
@@ -737,15 +823,13 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
 }
 
 int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, const int32_t* res, int32_t nchunks,
-                   const uw_mesh_out* out, uint32_t* head, uint32_t* order, unsigned char* ztab, hipStream_t st,
+                   const uw_mesh_out* out, uint32_t* head, void* task_buf, unsigned char* ztab, hipStream_t st,
                    uw_vertex_tape* tape = nullptr) {
     const uw_batch_tuning& tune = ctx->batch_tuning;
     BatchArgs a;
     memset(&a, 0, sizeof(a));
     a.f = uw_make_field(ctx->params);
     a.tab = ctx->d_tables;
-    a.coords = coords;
-    a.res = res;
     a.nchunks = nchunks;
     a.info = out->info;
     a.totals = out->totals;
@@ -762,12 +846,13 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.has_lioff = one_pass ? 0 : 1;
     a.axis_stride = tune.max_res + 1;
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
-    a.order = order;
+    uw_task* tasks = static_cast<uw_task*>(task_buf);
+    a.tasks = tasks;
     a.head = head;
     a.ztab = ztab;
     a.zstride = tune.max_res + 1;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, head, a.totals);
-    hipLaunchKernelGGL(k_order, dim3(1), dim3(1024), 0, st, res, nchunks, order);
+    hipLaunchKernelGGL(k_order, dim3(1), dim3(1024), 0, st, coords, res, nchunks, tasks, head, a.totals,
+                       uint32_t(bl.grid));
     if (int rc = uw_launch_ztables(ctx, a.f, coords, res, 0, nchunks, a.zstride, ztab, st)) return rc;
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
@@ -842,10 +927,12 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     UW_HIP(ctx, hipSetDevice(ctx->device));
     BatchLaunch bl;
     if (int rc = prepare_batched(ctx, nchunks, &bl)) return rc;
-    // the graph owns its queue head, order and z-table buffers: the context's may be reallocated
+    // the graph owns its queue head, task records and z tables (the context's may be reallocated):
+    // head at byte 0, records from byte 256, z tables after them
     uint32_t* gbuf = nullptr;
-    const size_t ord_bytes = ((size_t(nchunks) + 1) * sizeof(uint32_t) + 255) & ~size_t(255);
+    const size_t ord_bytes = (256 + size_t(nchunks) * sizeof(uw_task) + 255) & ~size_t(255);
     UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + bl.ztab_bytes));
+    unsigned char* gtasks = reinterpret_cast<unsigned char*>(gbuf) + 256;
     unsigned char* gztab = reinterpret_cast<unsigned char*>(gbuf) + ord_bytes;
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
@@ -857,7 +944,7 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     int rc = UW_OK;
     const bool was_profiling = ctx->profiling;
     ctx->profiling = false;  // no timing events inside a graph
-    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gbuf + 1, gztab, cs);
+    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gtasks, gztab, cs);
     ctx->profiling = was_profiling;
     hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &graph) : hipSuccess;
     (void)hipStreamDestroy(cs);

@@ -46,7 +46,7 @@ struct uw_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int device = 0;
-    uint32_t* buffers = nullptr;  // queue head + chunk order owned by the graph
+    uint32_t* buffers = nullptr;  // queue head, task records and z tables owned by the graph
 };
 
 void uw_default_batch_tuning(uw_batch_tuning* t);
@@ -57,7 +57,7 @@ struct uw_ctx {
     int num_cus = 256;
     uw_batch_tuning batch_tuning;
     uint32_t* d_batch_head = nullptr;
-    uint32_t* d_batch_order = nullptr;
+    void* d_batch_order = nullptr;  // task records of the batched queue
     size_t batch_order_cap = 0;
     unsigned char* d_batch_ztab = nullptr;  // per-chunk z tables of the eager batched path
     size_t batch_ztab_bytes = 0;

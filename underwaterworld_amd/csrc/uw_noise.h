@@ -153,28 +153,13 @@ __device__ __forceinline__ uw_line_oct uw_line_setup(const uw_axis& X, const uw_
     return s;
 }
 
-// Moves a line's state to the cell whose packed z bytes are zbits. When the new cell's low face
-// has the previous cell's high-face byte (the line stepped to the next cell, modulo 256, which
-// is all the noise sees of the index), that face is reused and only the new high face is built.
-__device__ __forceinline__ void uw_line_advance(uw_line_oct& s, uint32_t prev, const uw_axis& X, const uw_axis& Y,
-                                                uint32_t bb, uint32_t zbits, const float4* GP) {
-    if (prev != 0xFFFFFFFFu && (zbits & 255u) == ((prev >> 8) & 255u)) {
-        s.A0 = s.A1;
-        s.B0 = s.B1;
-        uw_line_face(X, Y, bb, (zbits >> 8) & 255u, GP, &s.A1, &s.B1);
-        uw_line_diff(s);
-    } else {
-        s = uw_line_setup(X, Y, bb, zbits, GP);
-    }
-}
-
 // Unscaled octave value at the line sample, clamped to the unscaled image of [-1, 1]; the caller
 // multiplies by the octave weight A_k (2/sqrt 3) / sum A.
 // N0 + s(dz) (N1 - N0), with the face difference N1 - N0 = C + D dz formed by one FMA.
-__device__ __forceinline__ float uw_line_sample(const uw_line_oct& s, const uw_axis& Z, float lo, float hi) {
-    const float n0 = fmaf(s.B0, Z.d, s.A0);
-    const float dn = fmaf(s.D, Z.d, s.C);
-    return __builtin_amdgcn_fmed3f(fmaf(Z.fade, dn, n0), lo, hi);
+__device__ __forceinline__ float uw_line_sample(const uw_line_oct& s, float dz, float fade, float lo, float hi) {
+    const float n0 = fmaf(s.B0, dz, s.A0);
+    const float dn = fmaf(s.D, dz, s.C);
+    return __builtin_amdgcn_fmed3f(fmaf(fade, dn, n0), lo, hi);
 }
 
 // Per-chunk LDS tables: every (axis index, octave) entry of the chunk and the z-bias split.
@@ -226,52 +211,107 @@ __device__ __forceinline__ void uw_build_xy_tables(const uw_field& f, const doub
 // Per-chunk z tables in HBM. Every lane of a wave walks its line at the same z index, so the z
 // entries and the bias split of a sample are wave-uniform: kept in HBM (written by a previous
 // kernel, uw_launch_ztables) they are fetched with scalar loads into scalar registers, costing
-// neither vector instructions nor vector registers. Layout per chunk, `stride` entries per row:
-//   az[k * stride + i]  (uw_axis, octave k, sample i), then adj[stride], post[stride] floats.
+// neither vector instructions nor vector registers. One record of uw_zrec_words(O) 32-bit words
+// per z sample, so a sample's fetch is one wide scalar load from one address:
+//   [3k] packed z lattice bytes of octave k (low face | high face << 8), [3k + 1] dz,
+//   [3k + 2] fade(dz); [3O] adj, [3O + 1] post (the z-bias split); [3O + 2] face flags: bit 2k,
+//   octave k enters a new lattice cell at this sample (every octave does at sample 0); bit 2k + 1,
+//   that cell's low face is the previous cell's high face (z byte stepped by one), so only the new
+//   high face is built.
+// Records 0 .. stride: the one past the last sample is zero and only ever fetched ahead.
+__host__ __device__ constexpr int32_t uw_zrec_words(int32_t octaves) { return (3 * octaves + 3 + 3) & ~3; }
+
 struct uw_ztab {
-    const uw_axis* az;
-    const float* adj;
-    const float* post;
-    int32_t stride;
+    const uint32_t* rec;
+    int32_t rw;  // words per record
 };
 
 __host__ __device__ inline size_t uw_ztab_bytes(int32_t octaves, int32_t stride) {
-    return (size_t(octaves) * size_t(stride) * sizeof(uw_axis) + 2 * size_t(stride) * sizeof(float) + 15) & ~size_t(15);
+    return size_t(stride + 1) * size_t(uw_zrec_words(octaves)) * sizeof(uint32_t);
 }
 
 __device__ __forceinline__ uw_ztab uw_ztab_of(const unsigned char* base, int32_t chunk, int32_t octaves,
                                               int32_t stride) {
     uw_ztab t;
-    const unsigned char* p = base + size_t(chunk) * uw_ztab_bytes(octaves, stride);
-    t.az = reinterpret_cast<const uw_axis*>(p);
-    t.adj = reinterpret_cast<const float*>(t.az + size_t(octaves) * stride);
-    t.post = t.adj + stride;
-    t.stride = stride;
+    t.rec = reinterpret_cast<const uint32_t*>(base + size_t(chunk) * uw_ztab_bytes(octaves, stride));
+    t.rw = uw_zrec_words(octaves);
     return t;
+}
+
+// Record i of a chunk's z table (sample coordinate cz, L samples, i <= L; i == L gives the zero
+// record).
+__device__ __forceinline__ void uw_fill_zrec(const uw_field& f, int32_t cz, int32_t L, float step, int32_t i,
+                                             uint32_t* rec) {
+    const int32_t O = f.octaves;
+    const int32_t rw = uw_zrec_words(O);
+    if (i >= L) {
+        for (int32_t w = 0; w < rw; ++w) rec[w] = 0u;
+        return;
+    }
+    const double pz = uw_sample_coord(f, 2, cz, i, step);
+    const double pp = i > 0 ? uw_sample_coord(f, 2, cz, i - 1, step) : 0.0;
+    uint32_t flags = 0;
+    for (int32_t k = 0; k < O; ++k) {
+        const uw_axis z = uw_axis_of(pz, f.octave_freq[k]);
+        rec[3 * k] = z.bits;
+        rec[3 * k + 1] = __float_as_uint(z.d);
+        rec[3 * k + 2] = __float_as_uint(z.fade);
+        if (i == 0) {
+            flags |= 1u << (2 * k);
+        } else {
+            const uint32_t prev = uw_axis_of(pp, f.octave_freq[k]).bits;
+            if (z.bits != prev) {
+                flags |= 1u << (2 * k);
+                if ((z.bits & 255u) == ((prev >> 8) & 255u)) flags |= 2u << (2 * k);
+            }
+        }
+    }
+    float adj, post;
+    uw_bias_terms(f, pz, &adj, &post);
+    rec[3 * O] = __float_as_uint(adj);
+    rec[3 * O + 1] = __float_as_uint(post);
+    rec[3 * O + 2] = flags;
+    for (int32_t w = 3 * O + 3; w < rw; ++w) rec[w] = 0u;
 }
 
 // Scalar (constant address space) loads of wave-uniform HBM data written by an earlier kernel.
 typedef const __attribute__((address_space(4))) uint32_t* uw_const_u32_ptr;
-__device__ __forceinline__ float uw_sload(const float* p) { return __uint_as_float(*(uw_const_u32_ptr)(p)); }
-__device__ __forceinline__ uw_axis uw_sload(const uw_axis* p) {
-    const uw_const_u32_ptr q = (uw_const_u32_ptr)(p);
-    uw_axis a;
-    a.bits = q[0];
-    a.d = __uint_as_float(q[1]);
-    a.dm1 = __uint_as_float(q[2]);
-    a.fade = __uint_as_float(q[3]);
-    return a;
+
+// Minimum / maximum of two finite values as single instructions (fminf / fmaxf also quiet
+// signalling NaN operands first, two more instructions per call in a loop).
+__device__ __forceinline__ float uw_fmin_raw(float a, float b) {
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ float uw_fmax_raw(float a, float b) {
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
 }
 
-// One chunk's z table (chunk coordinate cz, L samples per axis).
-__device__ __forceinline__ void uw_fill_ztab(const uw_field& f, int32_t cz, int32_t L, float step, uw_axis* az,
-                                             float* adj, float* post, int32_t stride, int tid, int nthr) {
-    for (int i = tid; i < L; i += nthr) {
-        const double pz = uw_sample_coord(f, 2, cz, i, step);
-        for (int32_t k = 0; k < f.octaves; ++k) az[k * stride + i] = uw_axis_of(pz, f.octave_freq[k]);
-        uw_bias_terms(f, pz, &adj[i], &post[i]);
+// One z record in scalar registers (OMAX octave slots, the first O used).
+template <int OMAX>
+struct uw_zsample {
+    uint32_t bits[OMAX];
+    float d[OMAX], fade[OMAX];
+    float adj, post;
+    uint32_t flags;
+    __device__ __forceinline__ void load(const uint32_t* rec, int32_t O) {
+        const uw_const_u32_ptr q = (uw_const_u32_ptr)(rec);
+#pragma unroll
+        for (int k = 0; k < OMAX; ++k) {
+            if (k < O) {
+                bits[k] = q[3 * k];
+                d[k] = __uint_as_float(q[3 * k + 1]);
+                fade[k] = __uint_as_float(q[3 * k + 2]);
+            }
+        }
+        adj = __uint_as_float(q[3 * O]);
+        post = __uint_as_float(q[3 * O + 1]);
+        flags = q[3 * O + 2];
     }
-}
+};
 
 // Sign summary bits of a set of samples: some sample not above iso (so not early-blank), some
 // below iso, some at or above iso (a cell can straddle iso only if both of the last two hold).
@@ -300,6 +340,7 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
     const int32_t O = uw_octaves<OCT>::count(f);
     const int32_t lines = (xb - xa) * L;
     const uw_divider dL = uw_make_divider(uint32_t(L)), dslot = uw_make_divider(uint32_t(nslot));
+    const int32_t rw = OCT > 0 ? uw_zrec_words(OCT) : Zt.rw;
     // running extremes of the thread's samples; the sign bits follow from them at the end
     float vmin = INFINITY, vmax = -INFINITY;
     for (int32_t t = tid; t < lines; t += nthr) {
@@ -308,7 +349,6 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
         float* out = dst + int32_t(uw_mod(uint32_t(ix), dslot)) * L * L + iy * L;
         uint32_t bb[OMAX];
         uw_line_oct st[OMAX];
-        uint32_t zcur[OMAX];
 #pragma unroll
         for (int k = 0; k < OMAX; ++k) {
             if (k < O) {
@@ -316,44 +356,52 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
                 const uint32_t a0 = xb2 & 255u, a1 = xb2 >> 8, y0 = yb & 255u, y1 = yb >> 8;
                 bb[k] = uint32_t(P[a0 ^ y0]) | (uint32_t(P[a0 ^ y1]) << 8) | (uint32_t(P[a1 ^ y0]) << 16) |
                         (uint32_t(P[a1 ^ y1]) << 24);
-                zcur[k] = 0xFFFFFFFFu;
             }
         }
-        // The z entries are wave-uniform: those of sample iz + 1 are fetched while sample iz is
-        // evaluated (the last fetch repeats index L - 1 and is discarded).
-        uw_axis Zn[OMAX];
-        float adjn = uw_sload(Zt.adj), postn = uw_sload(Zt.post);
+        // Sample iz, then the fetch of the record of sample iz + 1 (the fetch after the last sample
+        // reads the spare record and is discarded). The fetch is issued after the face updates,
+        // whose LDS reads would otherwise wait for it too, and before the sample arithmetic, which
+        // covers its latency. Two records alternate, so the fetched registers are used in place.
+        const uint32_t* zr = Zt.rec;  // record of the sample being evaluated
+        auto sample = [&](const uw_zsample<OMAX>& Z, uw_zsample<OMAX>& Znext, int32_t iz) {
+            if (Z.flags) {  // some octave enters a new lattice cell
 #pragma unroll
-        for (int k = 0; k < OMAX; ++k)
-            if (k < O) Zn[k] = uw_sload(Zt.az + k * Zt.stride);
-        for (int32_t iz = 0; iz < L; ++iz) {
-            uw_axis Z[OMAX];
-#pragma unroll
-            for (int k = 0; k < OMAX; ++k)
-                if (k < O) Z[k] = Zn[k];
-            const float adj = adjn, post = postn;
-            const int32_t izn = iz + 1 < L ? iz + 1 : iz;
-#pragma unroll
-            for (int k = 0; k < OMAX; ++k)
-                if (k < O) Zn[k] = uw_sload(Zt.az + k * Zt.stride + izn);
-            adjn = uw_sload(Zt.adj + izn);
-            postn = uw_sload(Zt.post + izn);
-            float total = 0.0f;
-#pragma unroll
-            for (int k = 0; k < OMAX; ++k) {
-                if (k < O) {
-                    if (Z[k].bits != zcur[k]) {
-                        uw_line_advance(st[k], zcur[k], T.AX[k * L + ix], T.AY[k * L + iy], bb[k], Z[k].bits, GP);
-                        zcur[k] = Z[k].bits;
+                for (int k = 0; k < OMAX; ++k) {
+                    if (k < O && (Z.flags & (1u << (2 * k)))) {
+                        const uw_axis X = T.AX[k * L + ix], Y = T.AY[k * L + iy];
+                        const uint32_t zb = Z.bits[k];
+                        if (Z.flags & (2u << (2 * k))) {
+                            st[k].A0 = st[k].A1;
+                            st[k].B0 = st[k].B1;
+                            uw_line_face(X, Y, bb[k], zb >> 8, GP, &st[k].A1, &st[k].B1);
+                            uw_line_diff(st[k]);
+                        } else {
+                            st[k] = uw_line_setup(X, Y, bb[k], zb, GP);
+                        }
                     }
-                    total = fmaf(f.weight[k], uw_line_sample(st[k], Z[k], f.clamp_lo, f.clamp_hi), total);
                 }
             }
-            const float rho = (adj + total) + post;
+            zr += rw;
+            Znext.load(zr, O);
+            __builtin_amdgcn_sched_barrier(0);  // keep the fetch ahead of the arithmetic
+            float total = 0.0f;
+#pragma unroll
+            for (int k = 0; k < OMAX; ++k)
+                if (k < O) total = fmaf(f.weight[k], uw_line_sample(st[k], Z.d[k], Z.fade[k], f.clamp_lo, f.clamp_hi),
+                                        total);
+            const float rho = (Z.adj + total) + Z.post;
             out[iz] = rho;
-            vmin = fminf(vmin, rho);
-            vmax = fmaxf(vmax, rho);
+            vmin = uw_fmin_raw(vmin, rho);
+            vmax = uw_fmax_raw(vmax, rho);
+        };
+        uw_zsample<OMAX> Za, Zb;
+        Za.load(zr, O);
+        int32_t iz = 0;
+        for (; iz + 1 < L; iz += 2) {  // no exit between a fetch and its use
+            sample(Za, Zb, iz);
+            sample(Zb, Za, iz + 1);
         }
+        if (iz < L) sample(Za, Zb, iz);
     }
     return (vmin <= f.iso ? UW_BELOW_NOT_ABOVE : 0) | (vmin < f.iso ? UW_BELOW_LT : 0) |
            (vmax >= f.iso ? UW_BELOW_GE : 0);
