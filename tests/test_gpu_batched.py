@@ -57,10 +57,32 @@ def test_batched_equals_plain_mixed():
     b = build_chunks_batched(perlin, coords, res)
     _sync()
     _assert_same(b, ref, len(coords))
+    _assert_totals(b, ref)
     t = b.totals()
-    assert t["overflow"] == 0
-    assert t["n_inds"] == sum(int(ref[i][0]["n_inds"]) for i in ref)
     print(f"\nmixed 128: nonblank {int(t['n_nonblank'])}, verts {int(t['n_verts'])}, inds {int(t['n_inds'])}")
+
+
+def _assert_totals(batch, ref):
+    t = batch.totals()
+    assert t["overflow"] == 0
+    assert t["n_verts"] == sum(int(ref[i][0]["n_verts"]) for i in ref)
+    assert t["n_inds"] == sum(int(ref[i][0]["n_inds"]) for i in ref)
+    assert t["n_nonblank"] == sum(int(ref[i][0]["n_inds"]) > 0 for i in ref)
+
+
+def test_batched_unpacked_reservations():
+    # 9000 chunks under a max_res-32 launch shape: the worst-case index total exceeds 2^32, so the
+    # launch reserves vertex and index ranges with separate atomics instead of one packed atomic
+    from underwaterworld_amd import Perlin, build_chunks_batched, get_batch_tuning
+    perlin = Perlin(PR.preset("mixed"))
+    assert get_batch_tuning(perlin)["max_res"] == 32
+    coords = [(x, y, z) for z in (-1, 0, 1) for y in range(-25, 25) for x in range(-30, 30)]
+    assert len(coords) == 9000 and 9000 * 15 * 32 ** 3 >= 2 ** 32
+    ref = _plain_by_resolution(perlin, coords, [8] * len(coords))
+    b = build_chunks_batched(perlin, coords, [8] * len(coords))
+    _sync()
+    _assert_same(b, ref, len(coords))
+    _assert_totals(b, ref)
 
 
 def test_batched_equals_plain_full_grid_12():
@@ -72,6 +94,7 @@ def test_batched_equals_plain_full_grid_12():
     b = build_chunks_batched(perlin, coords, 12)
     _sync()
     _assert_same(b, ref, len(coords))
+    _assert_totals(b, ref)
 
 
 @pytest.mark.parametrize("preset", ["reference", "mixed"])

@@ -29,9 +29,14 @@ constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
 constexpr int kRing = 2048;       // cell ring (vertex base << 8 | case), power of two
 constexpr int kMinList = 512;     // smallest compacted active-cell list
 constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
+// Queue state (one allocation, 512 bytes): the task head (u32) at byte 0, the vertex range counter
+// (u64; in packed launches vertices | indices << 32) at byte 128 and the index range counter (u64,
+// unpacked launches) at byte 256, each in its own cache line.
+constexpr size_t kQueueBytes = 512;
+constexpr int kResvIndsOffset = 16;  // u64 words from the vertex counter to the index counter
 
 // One queued chunk, in queue order (32 bytes, two 16-byte loads): chunk index c (-1: queue
-// empty), resolution n and chunk coordinates. Written by k_order, read once per task.
+// empty), resolution n and chunk coordinates. Written by k_prep, read once per task.
 struct __align__(16) uw_task {
     int32_t c, n, cx, cy;
     int32_t cz, pad[3];
@@ -43,6 +48,8 @@ struct BatchArgs {
     const uw_task* tasks;
     int32_t nchunks;
     uint32_t* head;
+    unsigned long long* resv;  // range counters (see kQueueBytes)
+    int32_t packed;            // 1: one packed atomic per reservation (worst-case totals fit 32 bits)
     uw_chunk_info* info;
     uw_totals* totals;
     float* verts;
@@ -88,31 +95,45 @@ __device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* cnt, int b, bool v
     return slot;
 }
 
-// Queue set-up (one workgroup): resets the output totals and the queue head (to first_task: the
-// kernel's workgroup g starts with task g without claiming it), and writes the task records in
-// queue order, a counting sort by resolution, largest first, invalid resolutions last.
-__global__ void __launch_bounds__(1024) k_order(const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
-                                                int32_t nchunks, uw_task* __restrict__ tasks, uint32_t* head,
-                                                uw_totals* totals, uint32_t first_task) {
-    __shared__ uint32_t cnt[kMaxN + 2];
-    if (threadIdx.x == 0) {
+constexpr int kPrepThreads = 256;
+constexpr int kOrderBatch = 8;  // chunks per thread whose loads are issued together
+
+__device__ __forceinline__ int res_bucket(int32_t n) { return (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1; }
+
+// Queue set-up (one workgroup): resets the output totals, the range counters and the queue head
+// (to first_task: the persistent kernel's workgroup g starts with task g without claiming it), and
+// writes the task records in queue order, a counting sort by resolution, largest first, invalid
+// resolutions last. Each thread issues the loads of kOrderBatch chunks before using any, so the
+// two passes wait on memory a few times rather than once per chunk.
+__device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
+                                            int32_t nchunks, uw_task* __restrict__ tasks, uint32_t* head,
+                                            uw_totals* totals, uint32_t first_task, uint32_t* cnt) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
         *head = first_task;
+        unsigned long long* resv = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned char*>(head) + 128);
+        resv[0] = 0ull;
+        resv[kResvIndsOffset] = 0ull;
         uw_totals z = {0, 0, 0, 0};
         *totals = z;
     }
-    for (int i = threadIdx.x; i < kMaxN + 2; i += blockDim.x) cnt[i] = 0;
+    for (int i = tid; i < kMaxN + 2; i += kPrepThreads) cnt[i] = 0;
     __syncthreads();
-    // the loops run whole waves (i0 uniform per wave) so the bucket adds can use ballots
-    const int32_t stride = int32_t(blockDim.x);
-    for (int32_t i0 = 0; i0 < nchunks; i0 += stride) {
-        const int32_t i = i0 + int32_t(threadIdx.x);
-        const bool valid = i < nchunks;
-        const int32_t n = valid ? res[i] : 0;
-        const int b = (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1;
-        wave_bucket_add(cnt, b, valid);
+    // whole waves run every step (the batch index is uniform), so the bucket adds can use ballots
+    constexpr int32_t kSpan = kPrepThreads * kOrderBatch;
+    for (int32_t base = 0; base < nchunks; base += kSpan) {
+        int32_t nb[kOrderBatch];
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const int32_t i = base + j * kPrepThreads + tid;
+            nb[j] = i < nchunks ? res[i] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j)
+            wave_bucket_add(cnt, res_bucket(nb[j]), base + j * kPrepThreads + tid < nchunks);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         uint32_t run = 0;
         for (int b = 0; b < kMaxN + 2; ++b) {
             const uint32_t c = cnt[b];
@@ -121,22 +142,92 @@ __global__ void __launch_bounds__(1024) k_order(const int32_t* __restrict__ coor
         }
     }
     __syncthreads();
-    for (int32_t i0 = 0; i0 < nchunks; i0 += stride) {
-        const int32_t i = i0 + int32_t(threadIdx.x);
-        const bool valid = i < nchunks;
-        const int32_t n = valid ? res[i] : 0;
-        const int b = (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1;
-        const uint32_t slot = wave_bucket_add(cnt, b, valid);
-        if (valid) {
-            uw_task t;
-            t.c = i;
-            t.n = n;
-            t.cx = coords[3 * i];
-            t.cy = coords[3 * i + 1];
-            t.cz = coords[3 * i + 2];
-            t.pad[0] = t.pad[1] = t.pad[2] = 0;
-            tasks[slot] = t;
+    for (int32_t base = 0; base < nchunks; base += kSpan) {
+        int32_t nb[kOrderBatch], xyz[kOrderBatch][3];
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const int32_t i = base + j * kPrepThreads + tid;
+            const bool valid = i < nchunks;
+            nb[j] = valid ? res[i] : 0;
+            xyz[j][0] = valid ? coords[3 * i] : 0;
+            xyz[j][1] = valid ? coords[3 * i + 1] : 0;
+            xyz[j][2] = valid ? coords[3 * i + 2] : 0;
         }
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) {
+            const int32_t i = base + j * kPrepThreads + tid;
+            const bool valid = i < nchunks;
+            const uint32_t slot = wave_bucket_add(cnt, res_bucket(nb[j]), valid);
+            if (valid) {
+                uw_task t;
+                t.c = i;
+                t.n = nb[j];
+                t.cx = xyz[j][0];
+                t.cy = xyz[j][1];
+                t.cz = xyz[j][2];
+                t.pad[0] = t.pad[1] = t.pad[2] = 0;
+                tasks[slot] = t;
+            }
+        }
+    }
+}
+
+// Launch set-up in one kernel: workgroup 0 orders the queue (order_tasks); the others write the
+// chunks' z tables, one thread per record (uw_ztab_record).
+__global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const int32_t* __restrict__ coords,
+                                                       const int32_t* __restrict__ res, int32_t nchunks,
+                                                       uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
+                                                       uint32_t first_task, int32_t stride,
+                                                       unsigned char* __restrict__ ztab) {
+    __shared__ uint32_t cnt[kMaxN + 2];
+    if (blockIdx.x == 0) {
+        order_tasks(coords, res, nchunks, tasks, head, totals, first_task, cnt);
+        return;
+    }
+    const uint64_t g = uint64_t(blockIdx.x - 1) * kPrepThreads + threadIdx.x;
+    uw_ztab_record(f, coords, res, 0, nchunks, stride, ztab, g);
+}
+
+// Reserves a chunk's contiguous vertex and index ranges (one thread). A packed launch (worst-case
+// totals below 2^32) takes both with one 64-bit atomic on vertices | indices << 32, halving the
+// traffic on the one contended counter; otherwise one atomic per range. The launch totals are
+// written after the kernel (k_finalize), so no other atomic goes to a shared counter per chunk.
+__device__ __forceinline__ void reserve_ranges(const BatchArgs& a, uint32_t nv, uint32_t ni, unsigned long long* vo,
+                                               unsigned long long* io, uint32_t* flags) {
+    if (a.packed) {
+        const unsigned long long old = atomicAdd(a.resv, (unsigned long long)nv | ((unsigned long long)ni << 32));
+        *vo = old & 0xFFFFFFFFull;
+        *io = old >> 32;
+    } else {
+        *vo = atomicAdd(a.resv, (unsigned long long)nv);
+        *io = atomicAdd(a.resv + kResvIndsOffset, (unsigned long long)ni);
+    }
+    if (*vo + nv > a.vcap || *io + ni > a.icap) {
+        *flags |= UW_CHUNK_OVERFLOW;
+        atomicOr(&a.totals->overflow, 1u);
+    }
+}
+
+// Launch totals, after the persistent kernel (one workgroup): the range counters hold the vertex
+// and index totals, and the non-blank chunks are those with indices. Writing them here keeps the
+// persistent kernel free of per-chunk or per-workgroup atomics on the totals, which, issued by
+// every workgroup as it finishes, queue on one counter at the end of the launch.
+__global__ void __launch_bounds__(1024) k_finalize(const unsigned long long* __restrict__ resv, int32_t packed,
+                                                   const uw_chunk_info* __restrict__ info, int32_t nchunks,
+                                                   uw_totals* totals) {
+    __shared__ uint32_t s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0u;
+    __syncthreads();
+    uint32_t cnt = 0;
+    for (int32_t i = threadIdx.x; i < nchunks; i += blockDim.x) cnt += info[i].n_inds > 0u ? 1u : 0u;
+    cnt = uw_wave_last(uw_wave_incl_scan(cnt));
+    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s_cnt, cnt);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long r0 = resv[0];
+        totals->n_verts = packed ? (r0 & 0xFFFFFFFFull) : r0;
+        totals->n_inds = packed ? (r0 >> 32) : resv[kResvIndsOffset];
+        totals->n_nonblank = s_cnt;
     }
 }
 
@@ -479,15 +570,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
             if (tid == 0) {
                 uint32_t flags = ni == 0 ? uint32_t(UW_CHUNK_BLANK) : 0u;
                 unsigned long long vo = 0, io = 0;
-                if (ni > 0) {
-                    vo = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_verts), (unsigned long long)nv);
-                    io = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_inds), (unsigned long long)ni);
-                    atomicAdd(&a.totals->n_nonblank, 1u);
-                    if (vo + nv > a.vcap || io + ni > a.icap) {
-                        flags |= UW_CHUNK_OVERFLOW;
-                        atomicOr(&a.totals->overflow, 1u);
-                    }
-                }
+                if (ni > 0) reserve_ranges(a, nv, ni, &vo, &io, &flags);
                 if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
                 uw_chunk_info r = {flags, nv, ni, na, vo, io};
                 a.info[c] = r;
@@ -619,15 +702,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
             unsigned long long vo = 0, io = 0;
             if (!below) flags = UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP;
             else if (ni == 0) flags = UW_CHUNK_BLANK;
-            if (ni > 0) {
-                vo = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_verts), (unsigned long long)nv);
-                io = atomicAdd(reinterpret_cast<unsigned long long*>(&a.totals->n_inds), (unsigned long long)ni);
-                atomicAdd(&a.totals->n_nonblank, 1u);
-                if (vo + nv > a.vcap || io + ni > a.icap) {
-                    flags |= UW_CHUNK_OVERFLOW;
-                    atomicOr(&a.totals->overflow, 1u);
-                }
-            }
+            if (ni > 0) reserve_ranges(a, nv, ni, &vo, &io, &flags);
             if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
             uw_chunk_info r = {flags, nv, ni, na, vo, io};
             a.info[c] = r;
@@ -795,7 +870,7 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, size_t(nchunks) * sizeof(uw_task)));
         ctx->batch_order_cap = size_t(nchunks);
     }
-    if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, sizeof(uint32_t)));
+    if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, kQueueBytes));
     bl->ztab_bytes = size_t(nchunks) * uw_ztab_bytes(octaves, tune.max_res + 1);
     if (bl->ztab_bytes > ctx->batch_ztab_bytes) {
         (void)hipFree(ctx->d_batch_ztab);
@@ -848,15 +923,22 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
     uw_task* tasks = static_cast<uw_task*>(task_buf);
     a.tasks = tasks;
+    a.resv = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned char*>(head) + 128);
+    // packed reservations need every launch total below 2^32 (chunks above max_res emit nothing)
+    const uint64_t worst_v = uint64_t(nchunks) * uw_max_verts(tune.max_res),
+                   worst_i = uint64_t(nchunks) * uw_max_inds(tune.max_res);
+    a.packed = (worst_v < (1ull << 32) && worst_i < (1ull << 32)) ? 1 : 0;
     a.head = head;
     a.ztab = ztab;
     a.zstride = tune.max_res + 1;
-    hipLaunchKernelGGL(k_order, dim3(1), dim3(1024), 0, st, coords, res, nchunks, tasks, head, a.totals,
-                       uint32_t(bl.grid));
-    if (int rc = uw_launch_ztables(ctx, a.f, coords, res, 0, nchunks, a.zstride, ztab, st)) return rc;
+    const uint64_t zblocks = (uint64_t(nchunks) * uint64_t(a.zstride + 1) + kPrepThreads - 1) / kPrepThreads;
+    if (zblocks + 1 > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
+    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks + 1)), dim3(kPrepThreads), 0, st, a.f, coords, res, nchunks, tasks,
+                       head, a.totals, uint32_t(bl.grid), a.zstride, ztab);
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, a.resv, a.packed, a.info, nchunks, a.totals);
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
 }
@@ -927,12 +1009,12 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     UW_HIP(ctx, hipSetDevice(ctx->device));
     BatchLaunch bl;
     if (int rc = prepare_batched(ctx, nchunks, &bl)) return rc;
-    // the graph owns its queue head, task records and z tables (the context's may be reallocated):
-    // head at byte 0, records from byte 256, z tables after them
+    // the graph owns its queue state, task records and z tables (the context's may be reallocated):
+    // queue state at byte 0, records after it, z tables after them
     uint32_t* gbuf = nullptr;
-    const size_t ord_bytes = (256 + size_t(nchunks) * sizeof(uw_task) + 255) & ~size_t(255);
+    const size_t ord_bytes = (kQueueBytes + size_t(nchunks) * sizeof(uw_task) + 255) & ~size_t(255);
     UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + bl.ztab_bytes));
-    unsigned char* gtasks = reinterpret_cast<unsigned char*>(gbuf) + 256;
+    unsigned char* gtasks = reinterpret_cast<unsigned char*>(gbuf) + kQueueBytes;
     unsigned char* gztab = reinterpret_cast<unsigned char*>(gbuf) + ord_bytes;
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {

@@ -274,6 +274,24 @@ __device__ __forceinline__ void uw_fill_zrec(const uw_field& f, int32_t cz, int3
     for (int32_t w = 3 * O + 3; w < rw; ++w) rec[w] = 0u;
 }
 
+// Record g of a launch's z tables (chunk g / (stride + 1), record g % (stride + 1)); res == null:
+// every chunk has n_uniform cells per axis. Records past a chunk's spare, and chunks whose
+// resolution does not fit the stride (flagged invalid by the mesher, which never reads them), are
+// left unwritten.
+__device__ __forceinline__ void uw_ztab_record(const uw_field& f, const int32_t* __restrict__ coords,
+                                               const int32_t* __restrict__ res, int32_t n_uniform, int32_t nchunks,
+                                               int32_t stride, unsigned char* __restrict__ out, uint64_t g) {
+    const uint32_t per = uint32_t(stride) + 1u;  // records per chunk
+    if (g >= uint64_t(nchunks) * per) return;
+    const int32_t c = int32_t(g / per), i = int32_t(g - uint64_t(c) * per);
+    const int32_t n = res ? res[c] : n_uniform;
+    if (n < 1 || n + 1 > stride) return;
+    if (i > n + 1) return;
+    uint32_t* rec = reinterpret_cast<uint32_t*>(out + size_t(c) * uw_ztab_bytes(f.octaves, stride)) +
+                    size_t(i) * size_t(uw_zrec_words(f.octaves));
+    uw_fill_zrec(f, coords[3 * c + 2], n + 1, float(f.extent) / float(n), i, rec);
+}
+
 // Scalar (constant address space) loads of wave-uniform HBM data written by an earlier kernel.
 typedef const __attribute__((address_space(4))) uint32_t* uw_const_u32_ptr;
 
