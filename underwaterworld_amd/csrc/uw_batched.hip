@@ -95,19 +95,23 @@ __device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* cnt, int b, bool v
     return slot;
 }
 
-constexpr int kPrepThreads = 256;
-constexpr int kOrderBatch = 8;  // chunks per thread whose loads are issued together
+constexpr int kPrepThreads = 1024;
+constexpr int kOrderBatch = 8;  // chunks per thread whose loads are in flight together
+static_assert(kMaxN + 2 <= 64, "the bucket prefix runs in one wave");
 
 __device__ __forceinline__ int res_bucket(int32_t n) { return (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1; }
 
-// Queue set-up (one workgroup): resets the output totals, the range counters and the queue head
-// (to first_task: the persistent kernel's workgroup g starts with task g without claiming it), and
-// writes the task records in queue order, a counting sort by resolution, largest first, invalid
-// resolutions last. Each thread issues the loads of kOrderBatch chunks before using any, so the
-// two passes wait on memory a few times rather than once per chunk.
+// Queue set-up (one workgroup of kPrepThreads): resets the output totals, the range counters and
+// the queue head (to first_task: the persistent kernel's workgroup g starts with task g without
+// claiming it), and writes the task records in queue order, a counting sort by resolution, largest
+// first, invalid resolutions last. Chunks go in batches of kPrepThreads * kOrderBatch: each thread
+// issues all loads of its kOrderBatch chunks before using any, and a launch of at most one batch
+// keeps them in registers from the counting pass to the placing pass (one wait on memory).
 __device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
                                             int32_t nchunks, uw_task* __restrict__ tasks, uint32_t* head,
                                             uw_totals* totals, uint32_t first_task, uint32_t* cnt) {
+    constexpr int NB = kMaxN + 2;  // buckets
+    constexpr int32_t kSpan = kPrepThreads * kOrderBatch;
     const int tid = threadIdx.x;
     if (tid == 0) {
         *head = first_task;
@@ -117,33 +121,9 @@ __device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, 
         uw_totals z = {0, 0, 0, 0};
         *totals = z;
     }
-    for (int i = tid; i < kMaxN + 2; i += kPrepThreads) cnt[i] = 0;
-    __syncthreads();
-    // whole waves run every step (the batch index is uniform), so the bucket adds can use ballots
-    constexpr int32_t kSpan = kPrepThreads * kOrderBatch;
-    for (int32_t base = 0; base < nchunks; base += kSpan) {
-        int32_t nb[kOrderBatch];
-#pragma unroll
-        for (int j = 0; j < kOrderBatch; ++j) {
-            const int32_t i = base + j * kPrepThreads + tid;
-            nb[j] = i < nchunks ? res[i] : 0;
-        }
-#pragma unroll
-        for (int j = 0; j < kOrderBatch; ++j)
-            wave_bucket_add(cnt, res_bucket(nb[j]), base + j * kPrepThreads + tid < nchunks);
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t run = 0;
-        for (int b = 0; b < kMaxN + 2; ++b) {
-            const uint32_t c = cnt[b];
-            cnt[b] = run;
-            run += c;
-        }
-    }
-    __syncthreads();
-    for (int32_t base = 0; base < nchunks; base += kSpan) {
-        int32_t nb[kOrderBatch], xyz[kOrderBatch][3];
+    if (tid < NB) cnt[tid] = 0;
+    int32_t nb[kOrderBatch], xyz[kOrderBatch][3];
+    auto load = [&](int32_t base) {
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j) {
             const int32_t i = base + j * kPrepThreads + tid;
@@ -153,6 +133,25 @@ __device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, 
             xyz[j][1] = valid ? coords[3 * i + 1] : 0;
             xyz[j][2] = valid ? coords[3 * i + 2] : 0;
         }
+    };
+    load(0);
+    __syncthreads();
+    // whole waves run every step (the batch index is uniform), so the bucket adds can use ballots
+    for (int32_t base = 0; base < nchunks; base += kSpan) {
+        if (base > 0) load(base);
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j)
+            wave_bucket_add(cnt, res_bucket(nb[j]), base + j * kPrepThreads + tid < nchunks);
+    }
+    __syncthreads();
+    if (tid < 64) {  // exclusive prefix over the buckets, one wave
+        const uint32_t v = tid < NB ? cnt[tid] : 0u;
+        const uint32_t inc = uw_wave_incl_scan(v);
+        if (tid < NB) cnt[tid] = inc - v;
+    }
+    __syncthreads();
+    for (int32_t base = 0; base < nchunks; base += kSpan) {
+        if (nchunks > kSpan) load(base);
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j) {
             const int32_t i = base + j * kPrepThreads + tid;
