@@ -75,42 +75,42 @@ __global__ void k_reset(uint32_t* head, uw_totals* totals) {
     }
 }
 
-// Adds the wave's lanes whose bucket equals b to cnt[b], one LDS atomic per distinct bucket in
-// the wave, and returns each valid lane's slot: the bucket's old count plus its rank among the
-// wave's lanes of that bucket.
-__device__ __forceinline__ uint32_t wave_bucket_add(uint32_t* cnt, int b, bool valid) {
-    const int lane = threadIdx.x & 63;
-    uint64_t todo = __ballot(valid);
-    uint32_t slot = 0;
-    while (todo) {
-        const int lead = __ffsll((unsigned long long)todo) - 1;
-        const int bl = __shfl(b, lead);
-        const uint64_t m = __ballot(valid && b == bl);
-        uint32_t base = 0;
-        if (lane == lead) base = atomicAdd(&cnt[bl], uint32_t(__popcll(m)));
-        base = __shfl(base, lead);
-        if (valid && b == bl) slot = base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
-        todo &= ~m;
-    }
-    return slot;
-}
-
 constexpr int kPrepThreads = 1024;
 constexpr int kOrderBatch = 8;  // chunks per thread whose loads are in flight together
-static_assert(kMaxN + 2 <= 64, "the bucket prefix runs in one wave");
+constexpr int kSurfBins = 8;                          // surface-distance classes per resolution
+constexpr int kOrderBuckets = (kMaxN + 2) * kSurfBins;  // queue-order buckets
+static_assert(kOrderBuckets <= kPrepThreads, "one thread per bucket in the prefix");
 
-__device__ __forceinline__ int res_bucket(int32_t n) { return (n >= 1 && n <= kMaxN) ? kMaxN - n : kMaxN + 1; }
+// Queue-order bucket of a chunk: resolution first (largest first, invalid last), then the distance
+// of the z-bias across the chunk from the isovalue (nearest first). The density is the bias plus
+// an fBm of magnitude at most 1, so a chunk whose bias range lies near the isovalue is the one
+// likely to hold surface, and to cost a mesh on top of its densities; chunks far from it are
+// blank or solid and cost the densities alone. Placing the costly ones first (longest task first)
+// leaves cheap tasks for the end of the launch, when the last round's balance sets the kernel
+// time. Only the processing order depends on it, never a result.
+__device__ __forceinline__ int order_bucket(const uw_field& f, int32_t n, int32_t cz) {
+    if (n < 1 || n > kMaxN) return (kMaxN + 1) * kSurfBins;
+    const float step = float(f.extent) / float(n);
+    float a0, p0, a1, p1;
+    uw_bias_terms(f, uw_sample_coord(f, 2, cz, 0, step), &a0, &p0);
+    uw_bias_terms(f, uw_sample_coord(f, 2, cz, n, step), &a1, &p1);
+    const float b0 = (a0 + p0) - f.iso, b1 = (a1 + p1) - f.iso;
+    // distance of [min(b0, b1), max(b0, b1)] from 0, in eighths
+    const float d = (b0 > 0.0f && b1 > 0.0f) ? fminf(b0, b1) : ((b0 < 0.0f && b1 < 0.0f) ? -fmaxf(b0, b1) : 0.0f);
+    const int k = d < 1.0f ? int(d * 8.0f) : kSurfBins - 1;
+    return (kMaxN - n) * kSurfBins + (k < kSurfBins ? k : kSurfBins - 1);
+}
 
 // Queue set-up (one workgroup of kPrepThreads): resets the output totals, the range counters and
 // the queue head (to first_task: the persistent kernel's workgroup g starts with task g without
-// claiming it), and writes the task records in queue order, a counting sort by resolution, largest
-// first, invalid resolutions last. Chunks go in batches of kPrepThreads * kOrderBatch: each thread
+// claiming it), and writes the task records in queue order, a counting sort by order_bucket
+// (resolution, largest first and invalid last, then nearest to the surface first). Chunks go in batches of kPrepThreads * kOrderBatch: each thread
 // issues all loads of its kOrderBatch chunks before using any, and a launch of at most one batch
 // keeps them in registers from the counting pass to the placing pass (one wait on memory).
-__device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
-                                            int32_t nchunks, uw_task* __restrict__ tasks, uint32_t* head,
-                                            uw_totals* totals, uint32_t first_task, uint32_t* cnt) {
-    constexpr int NB = kMaxN + 2;  // buckets
+__device__ __forceinline__ void order_tasks(const uw_field& f, const int32_t* __restrict__ coords,
+                                            const int32_t* __restrict__ res, int32_t nchunks,
+                                            uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
+                                            uint32_t first_task, uint32_t* cnt, uint32_t* wsum) {
     constexpr int32_t kSpan = kPrepThreads * kOrderBatch;
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -121,8 +121,8 @@ __device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, 
         uw_totals z = {0, 0, 0, 0};
         *totals = z;
     }
-    if (tid < NB) cnt[tid] = 0;
-    int32_t nb[kOrderBatch], xyz[kOrderBatch][3];
+    if (tid < kOrderBuckets) cnt[tid] = 0;
+    int32_t nb[kOrderBatch], xyz[kOrderBatch][3], bk[kOrderBatch];
     auto load = [&](int32_t base) {
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j) {
@@ -133,21 +133,28 @@ __device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, 
             xyz[j][1] = valid ? coords[3 * i + 1] : 0;
             xyz[j][2] = valid ? coords[3 * i + 2] : 0;
         }
+#pragma unroll
+        for (int j = 0; j < kOrderBatch; ++j) bk[j] = order_bucket(f, nb[j], xyz[j][2]);
     };
     load(0);
     __syncthreads();
-    // whole waves run every step (the batch index is uniform), so the bucket adds can use ballots
+    // bucket counts and, below, slots: one LDS atomic per chunk (lanes of one bucket serialise in
+    // the LDS, and a wave holds chunks of many buckets)
     for (int32_t base = 0; base < nchunks; base += kSpan) {
         if (base > 0) load(base);
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j)
-            wave_bucket_add(cnt, res_bucket(nb[j]), base + j * kPrepThreads + tid < nchunks);
+            if (base + j * kPrepThreads + tid < nchunks) atomicAdd(&cnt[bk[j]], 1u);
     }
     __syncthreads();
-    if (tid < 64) {  // exclusive prefix over the buckets, one wave
-        const uint32_t v = tid < NB ? cnt[tid] : 0u;
+    {  // exclusive prefix over the buckets: one bucket per thread, wave scans, then wave offsets
+        const uint32_t v = tid < kOrderBuckets ? cnt[tid] : 0u;
         const uint32_t inc = uw_wave_incl_scan(v);
-        if (tid < NB) cnt[tid] = inc - v;
+        if ((tid & 63) == 63) wsum[tid >> 6] = inc;
+        __syncthreads();
+        uint32_t off = 0;
+        for (int w = 0; w < (tid >> 6); ++w) off += wsum[w];
+        if (tid < kOrderBuckets) cnt[tid] = off + inc - v;
     }
     __syncthreads();
     for (int32_t base = 0; base < nchunks; base += kSpan) {
@@ -155,9 +162,8 @@ __device__ __forceinline__ void order_tasks(const int32_t* __restrict__ coords, 
 #pragma unroll
         for (int j = 0; j < kOrderBatch; ++j) {
             const int32_t i = base + j * kPrepThreads + tid;
-            const bool valid = i < nchunks;
-            const uint32_t slot = wave_bucket_add(cnt, res_bucket(nb[j]), valid);
-            if (valid) {
+            if (i < nchunks) {
+                const uint32_t slot = atomicAdd(&cnt[bk[j]], 1u);
                 uw_task t;
                 t.c = i;
                 t.n = nb[j];
@@ -178,9 +184,10 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const int32_t
                                                        uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
                                                        uint32_t first_task, int32_t stride,
                                                        unsigned char* __restrict__ ztab) {
-    __shared__ uint32_t cnt[kMaxN + 2];
+    __shared__ uint32_t cnt[kOrderBuckets];
+    __shared__ uint32_t wsum[kPrepThreads / 64];
     if (blockIdx.x == 0) {
-        order_tasks(coords, res, nchunks, tasks, head, totals, first_task, cnt);
+        order_tasks(f, coords, res, nchunks, tasks, head, totals, first_task, cnt, wsum);
         return;
     }
     const uint64_t g = uint64_t(blockIdx.x - 1) * kPrepThreads + threadIdx.x;
