@@ -224,8 +224,18 @@ __global__ void __launch_bounds__(1024) k_finalize(const unsigned long long* __r
     __shared__ uint32_t s_cnt;
     if (threadIdx.x == 0) s_cnt = 0u;
     __syncthreads();
+    // loads of up to 8 records per thread in flight together
     uint32_t cnt = 0;
-    for (int32_t i = threadIdx.x; i < nchunks; i += blockDim.x) cnt += info[i].n_inds > 0u ? 1u : 0u;
+    for (int32_t base = 0; base < nchunks; base += 8 * 1024) {
+        uint32_t ni[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int32_t i = base + j * 1024 + int32_t(threadIdx.x);
+            ni[j] = i < nchunks ? info[i].n_inds : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) cnt += ni[j] > 0u ? 1u : 0u;
+    }
     cnt = uw_wave_last(uw_wave_incl_scan(cnt));
     if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s_cnt, cnt);
     __syncthreads();
