@@ -55,7 +55,9 @@ struct BatchArgs {
     float* verts;
     uint32_t* inds;
     uw_vertex_tape* tape;  // optional per-vertex record for the adjoint
-    const unsigned char* ztab;  // per-chunk z tables (uw_ztab layout), written by k_ztables
+    const unsigned char* ztab;  // per-chunk z tables (uw_ztab layout), written by k_prep
+    float* dscr;                // per-workgroup density scratch for x-slab chunks (null: recompute)
+    uint64_t dscr_stride;       // floats per workgroup: (max_res + 1)^3
     int32_t zstride;            // samples per z-table row: max_res + 1
     uw_emit_opts eo;       // vertex / index formats and optional collision records
     uint64_t vcap, icap;
@@ -698,6 +700,16 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                      UW_BELOW_NOT_ABOVE;
 
             __syncthreads();
+            if (nslab > 1 && a.dscr) {
+                // the slab's new planes also go to the workgroup's scratch, for the emission pass
+                float* gs = a.dscr + uint64_t(blockIdx.x) * a.dscr_stride;
+                const int32_t LL = L * L;
+                for (int32_t px = s == 0 ? 0 : xa + 1; px <= xe; ++px) {
+                    const float* src = DENS + int32_t(uw_mod(uint32_t(px), dns)) * LL;
+                    float* dst = gs + px * LL;
+                    for (int32_t r = tid; r < LL; r += BLOCK) dst[r] = src[r];
+                }
+            }
             for (int32_t g = xa * nn + tid; g < xe * nn; g += BLOCK) {
                 int32_t x, y, z;
                 uw_cell_xyz(uint32_t(g), dn, dnn, &x, &y, &z);
@@ -738,7 +750,17 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
             if (nslab > 1) {
-                uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+                if (a.dscr) {  // the counting pass's densities, back from the scratch
+                    const float* gs = a.dscr + uint64_t(blockIdx.x) * a.dscr_stride;
+                    const int32_t LL = L * L;
+                    for (int32_t px = s == 0 ? 0 : xa + 1; px <= xe; ++px) {
+                        const float* src = gs + px * LL;
+                        float* dst = DENS + int32_t(uw_mod(uint32_t(px), dns)) * LL;
+                        for (int32_t r = tid; r < LL; r += BLOCK) dst[r] = src[r];
+                    }
+                } else {
+                    uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+                }
                 __syncthreads();
             }
             uint32_t list_n = 0;
@@ -874,6 +896,8 @@ struct BatchLaunch {
     size_t lds;
     int grid;
     size_t ztab_bytes;  // z tables of nchunks chunks
+    uint64_t dscr_stride;  // density scratch floats per workgroup (0: no resolution of the shape has x-slabs)
+    size_t dscr_bytes;     // density scratch of the whole grid
 };
 
 int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
@@ -910,12 +934,26 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         if (per_cu < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "batched kernel does not fit on a CU with this tuning");
         bl->grid = per_cu * ctx->num_cus;
     }
+    // x-slab chunks keep their counting-pass densities in a per-workgroup scratch for the emission
+    // pass (4 (n+1)^3 bytes of HBM traffic each way instead of a second density evaluation)
+    bool slabs = false;
+    for (int n = 1; n <= tune.max_res; ++n) slabs = slabs || tune.slab[n] < n;
+    const uint64_t Lm = uint64_t(tune.max_res) + 1;
+    bl->dscr_stride = slabs ? Lm * Lm * Lm : 0;
+    bl->dscr_bytes = size_t(bl->dscr_stride) * size_t(bl->grid) * sizeof(float);
+    if (bl->dscr_bytes > ctx->batch_dscr_bytes) {
+        (void)hipFree(ctx->d_batch_dscr);
+        ctx->d_batch_dscr = nullptr;
+        ctx->batch_dscr_bytes = 0;
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_dscr, bl->dscr_bytes));
+        ctx->batch_dscr_bytes = bl->dscr_bytes;
+    }
     return UW_OK;
 }
 
 int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, const int32_t* res, int32_t nchunks,
-                   const uw_mesh_out* out, uint32_t* head, void* task_buf, unsigned char* ztab, hipStream_t st,
-                   uw_vertex_tape* tape = nullptr) {
+                   const uw_mesh_out* out, uint32_t* head, void* task_buf, unsigned char* ztab, float* dscr,
+                   hipStream_t st, uw_vertex_tape* tape = nullptr) {
     const uw_batch_tuning& tune = ctx->batch_tuning;
     BatchArgs a;
     memset(&a, 0, sizeof(a));
@@ -946,6 +984,8 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.packed = (worst_v < (1ull << 32) && worst_i < (1ull << 32)) ? 1 : 0;
     a.head = head;
     a.ztab = ztab;
+    a.dscr = bl.dscr_stride ? dscr : nullptr;
+    a.dscr_stride = bl.dscr_stride;
     a.zstride = tune.max_res + 1;
     const uint64_t zblocks = (uint64_t(nchunks) * uint64_t(a.zstride + 1) + kPrepThreads - 1) / kPrepThreads;
     if (zblocks + 1 > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
@@ -983,7 +1023,7 @@ int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* r
         return UW_OK;
     }
     return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, ctx->d_batch_ztab,
-                          st);
+                          ctx->d_batch_dscr, st);
 }
 
 int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
@@ -1001,7 +1041,7 @@ int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32
         return UW_OK;
     }
     return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, ctx->d_batch_ztab,
-                          st, tape);
+                          ctx->d_batch_dscr, st, tape);
 }
 
 int uw_get_batch_tuning(const uw_ctx* ctx, uw_batch_tuning* t) {
@@ -1029,9 +1069,11 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     // queue state at byte 0, records after it, z tables after them
     uint32_t* gbuf = nullptr;
     const size_t ord_bytes = (kQueueBytes + size_t(nchunks) * sizeof(uw_task) + 255) & ~size_t(255);
-    UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + bl.ztab_bytes));
+    const size_t ztab_bytes = (bl.ztab_bytes + 255) & ~size_t(255);
+    UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + ztab_bytes + bl.dscr_bytes));
     unsigned char* gtasks = reinterpret_cast<unsigned char*>(gbuf) + kQueueBytes;
     unsigned char* gztab = reinterpret_cast<unsigned char*>(gbuf) + ord_bytes;
+    float* gdscr = reinterpret_cast<float*>(gztab + ztab_bytes);
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(gbuf);
@@ -1042,7 +1084,7 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     int rc = UW_OK;
     const bool was_profiling = ctx->profiling;
     ctx->profiling = false;  // no timing events inside a graph
-    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gtasks, gztab, cs);
+    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gtasks, gztab, gdscr, cs);
     ctx->profiling = was_profiling;
     hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &graph) : hipSuccess;
     (void)hipStreamDestroy(cs);

@@ -61,6 +61,8 @@ struct uw_ctx {
     size_t batch_order_cap = 0;
     unsigned char* d_batch_ztab = nullptr;  // per-chunk z tables of the eager batched path
     size_t batch_ztab_bytes = 0;
+    float* d_batch_dscr = nullptr;  // per-workgroup density scratch of x-slab chunks (eager batched path)
+    size_t batch_dscr_bytes = 0;
     uint8_t perm[256];
     uw_tables host_tables;
     uw_tables* d_tables = nullptr;
