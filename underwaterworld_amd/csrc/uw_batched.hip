@@ -323,7 +323,7 @@ __device__ __forceinline__ CellCounts classify_cell(const float* ring, uw_divide
 
 // LDS carve-up (bytes): fixed part, then the per-octave axis tables, then the density ring.
 __host__ __device__ constexpr size_t lds_fixed_bytes(size_t list_cap, size_t ring_cap, int has_lioff) {
-    return 256 * 16 /*GP*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + 256 * 12 * 2 /*BEF*/ + ring_cap * 4 +
+    return 256 * 12 /*G3*/ + 256 * 8 /*TRI*/ + 256 * 4 /*LUT*/ + 256 * 12 * 2 /*BEF*/ + ring_cap * 4 +
            list_cap * 4 * (has_lioff ? 2 : 1) + 256 /*P*/;
 }
 __host__ __device__ inline size_t lds_table_bytes(int octaves, int axis_stride) {
@@ -432,8 +432,8 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     constexpr int W = BLOCK / 64;
     extern __shared__ __align__(16) unsigned char smem[];
     const int32_t O = a.f.octaves;
-    float4* GP = reinterpret_cast<float4*>(smem);
-    uint64_t* TRI = reinterpret_cast<uint64_t*>(GP + 256);
+    float* G3 = reinterpret_cast<float*>(smem);  // gradient of each permutation slot, 3 floats
+    uint64_t* TRI = reinterpret_cast<uint64_t*>(G3 + 3 * 256);
     uint32_t* LUT = reinterpret_cast<uint32_t*>(TRI + 256);
     uint16_t* BEF = reinterpret_cast<uint16_t*>(LUT + 256);  // first-seen rank masks [256][12]
     uint32_t* RING = reinterpret_cast<uint32_t*>(BEF + 256 * 12);
@@ -456,7 +456,10 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     for (int i = tid; i < 256; i += BLOCK) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
-        GP[i] = tab->grad[p & 15];
+        const float4 g = tab->grad[p & 15];
+        G3[3 * i] = g.x;
+        G3[3 * i + 1] = g.y;
+        G3[3 * i + 2] = g.z;
         TRI[i] = tab->tri[i];
         LUT[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
     }
@@ -517,7 +520,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
 
         if (nslab == 1 && C <= a.list_cap) {
             // ---------- one density evaluation, one classification pass ----------
-            const int bits = uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, 0, L, DENS, ns, tid, BLOCK);
+            const int bits = uw_noise_planes<OCT>(a.f, T, Zt, P, G3, L, 0, L, DENS, ns, tid, BLOCK);
             // s_next was read before the table barrier; the next read follows the end-of-task barrier
             if (lookahead) {
                 if (pf) stage_task(a, atomicAdd(a.head, 1u), s_next);
@@ -696,7 +699,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         unsigned long long cnt = 0;  // nv | nt << 24 | act << 48 (per thread)
         for (int32_t s = 0; s < nslab; ++s) {
             const int32_t xa = s * sx, xe = min(n, xa + sx);
-            below |= uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
+            below |= uw_noise_planes<OCT>(a.f, T, Zt, P, G3, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK) &
                      UW_BELOW_NOT_ABOVE;
 
             __syncthreads();
@@ -759,7 +762,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                         for (int32_t r = tid; r < LL; r += BLOCK) dst[r] = src[r];
                     }
                 } else {
-                    uw_noise_planes<OCT>(a.f, T, Zt, P, GP, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
+                    uw_noise_planes<OCT>(a.f, T, Zt, P, G3, L, s == 0 ? 0 : xa + 1, xe + 1, DENS, ns, tid, BLOCK);
                 }
                 __syncthreads();
             }

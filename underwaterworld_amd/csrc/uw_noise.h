@@ -129,13 +129,22 @@ __device__ __forceinline__ void uw_line_diff(uw_line_oct& s) {
     s.D = s.B1 - s.B0;
 }
 
+// Gradient of permutation slot j (the vector of code P[j] & 15) from a packed LDS table of three
+// floats per slot: with a 12-byte stride the slots of a gather spread over all LDS banks (a 16-byte
+// stride puts each component of every slot on the same 8 of the 32 banks).
+__device__ __forceinline__ float4 uw_grad3(const float* G3, uint32_t j) {
+    const float* g = G3 + 3u * j;
+    return make_float4(g[0], g[1], g[2], 0.0f);
+}
+
 // One z face of a line's lattice cell: A = lerp_y(lerp_x(P_..)), B = lerp_y(lerp_x(Gz_..)) for the
 // face whose z lattice byte is zb. bb = b00 | b01 << 8 | b10 << 16 | b11 << 24 (second-level
 // hashes of the line's x/y corners). The face depends on the z lattice index only through zb.
 __device__ __forceinline__ void uw_line_face(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zb,
-                                             const float4* GP, float* A, float* B) {
+                                             const float* G3, float* A, float* B) {
     const uint32_t b00 = bb & 255u, b01 = (bb >> 8) & 255u, b10 = (bb >> 16) & 255u, b11 = bb >> 24;
-    const float4 G00 = GP[b00 ^ zb], G10 = GP[b10 ^ zb], G01 = GP[b01 ^ zb], G11 = GP[b11 ^ zb];
+    const float4 G00 = uw_grad3(G3, b00 ^ zb), G10 = uw_grad3(G3, b10 ^ zb), G01 = uw_grad3(G3, b01 ^ zb),
+                 G11 = uw_grad3(G3, b11 ^ zb);
     const float P00 = fmaf(G00.y, Y.d, G00.x * X.d), P10 = fmaf(G10.y, Y.d, G10.x * X.dm1);
     const float P01 = fmaf(G01.y, Y.dm1, G01.x * X.d), P11 = fmaf(G11.y, Y.dm1, G11.x * X.dm1);
     const float a = X.fade, b = Y.fade;
@@ -145,10 +154,10 @@ __device__ __forceinline__ void uw_line_face(const uw_axis& X, const uw_axis& Y,
 
 // Per-line, per-octave state for the lattice cell whose low/high z bytes are packed in zbits.
 __device__ __forceinline__ uw_line_oct uw_line_setup(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zbits,
-                                                     const float4* GP) {
+                                                     const float* G3) {
     uw_line_oct s;
-    uw_line_face(X, Y, bb, zbits & 255u, GP, &s.A0, &s.B0);
-    uw_line_face(X, Y, bb, (zbits >> 8) & 255u, GP, &s.A1, &s.B1);
+    uw_line_face(X, Y, bb, zbits & 255u, G3, &s.A0, &s.B0);
+    uw_line_face(X, Y, bb, (zbits >> 8) & 255u, G3, &s.A1, &s.B1);
     uw_line_diff(s);
     return s;
 }
@@ -352,7 +361,7 @@ struct uw_octaves {
 // Returns, per thread, the OR of the UW_BELOW_* bits of its samples.
 template <int OCT>
 __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uw_ztab& Zt,
-                                               const uint8_t* P, const float4* GP, int32_t L, int32_t xa, int32_t xb,
+                                               const uint8_t* P, const float* G3, int32_t L, int32_t xa, int32_t xb,
                                                float* dst, int32_t nslot, int tid, int nthr) {
     constexpr int OMAX = uw_octaves<OCT>::kMax;
     const int32_t O = uw_octaves<OCT>::count(f);
@@ -391,10 +400,10 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
                         if (Z.flags & (2u << (2 * k))) {
                             st[k].A0 = st[k].A1;
                             st[k].B0 = st[k].B1;
-                            uw_line_face(X, Y, bb[k], zb >> 8, GP, &st[k].A1, &st[k].B1);
+                            uw_line_face(X, Y, bb[k], zb >> 8, G3, &st[k].A1, &st[k].B1);
                             uw_line_diff(st[k]);
                         } else {
-                            st[k] = uw_line_setup(X, Y, bb[k], zb, GP);
+                            st[k] = uw_line_setup(X, Y, bb[k], zb, G3);
                         }
                     }
                 }

@@ -41,8 +41,8 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     const int32_t chunk = blockIdx.x / slabs, slab = blockIdx.x % slabs;
     const int32_t x0 = slab * planes;
     const int32_t nx = min(planes, L - x0);
-    float4* GP = reinterpret_cast<float4*>(smem);                  // 256 x 16 B
-    uint8_t* P = reinterpret_cast<uint8_t*>(GP + 256);             // 256 B
+    float* G3 = reinterpret_cast<float*>(smem);                    // 256 x 3 floats
+    uint8_t* P = reinterpret_cast<uint8_t*>(G3 + 3 * 256);         // 256 B
     uw_axis* AX = reinterpret_cast<uw_axis*>(P + 256);             // [O][L] x2
     uw_axis* AY = AX + O * L;
     float* SLAB = reinterpret_cast<float*>(AY + O * L);            // [planes][L][L]
@@ -55,7 +55,10 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
-        GP[i] = tab->grad[p & 15];
+        const float4 g = tab->grad[p & 15];
+        G3[3 * i] = g.x;
+        G3[3 * i + 1] = g.y;
+        G3[3 * i + 2] = g.z;
     }
     __syncthreads();
     const uw_chunk_tables T = {AX, AY, nullptr, nullptr, nullptr};
@@ -63,7 +66,7 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     const uw_ztab Zt = uw_ztab_of(ztab, chunk, O, L);
     __syncthreads();
     // plane x lands in slot x % planes = x - x0
-    const int bits = uw_noise_planes<OCT>(f, T, Zt, P, GP, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
+    const int bits = uw_noise_planes<OCT>(f, T, Zt, P, G3, L, x0, x0 + nx, SLAB, planes, threadIdx.x, blockDim.x);
     if (bits) atomicOr(&s_bits, uint32_t(bits));
     __syncthreads();
     const int32_t LL = L * L;
@@ -313,7 +316,7 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
 
 // LDS bytes of k_noise_grid.
 size_t noise_lds_bytes(int32_t O, int32_t planes, int32_t L) {
-    return 256 * sizeof(float4) + 256 + sizeof(uw_axis) * 2 * size_t(O) * size_t(L) +
+    return 256 * 3 * sizeof(float) + 256 + sizeof(uw_axis) * 2 * size_t(O) * size_t(L) +
            sizeof(float) * size_t(planes) * size_t(L) * size_t(L);
 }
 
