@@ -28,14 +28,16 @@ constexpr int kBlock = 256;
 // Octave value and gradient with respect to the octave coordinate (f32), given the per-axis
 // lattice data and fade derivatives s'(d) = 30 d^2 (d - 1)^2.
 __device__ __forceinline__ float uw_octave_grad(const uw_axis& X, const uw_axis& Y, const uw_axis& Z, float dfx,
-                                                float dfy, float dfz, const uint8_t* P, const float4* GP,
+                                                float dfy, float dfz, const uint8_t* P, const float* G3,
                                                 float g[3]) {
     const uint32_t a0 = X.bits & 255, a1 = X.bits >> 8;
     const uint32_t y0 = Y.bits & 255, y1 = Y.bits >> 8;
     const uint32_t z0 = Z.bits & 255, z1 = Z.bits >> 8;
     const uint32_t b00 = P[a0 ^ y0], b01 = P[a0 ^ y1], b10 = P[a1 ^ y0], b11 = P[a1 ^ y1];
-    const float4 G000 = GP[b00 ^ z0], G100 = GP[b10 ^ z0], G010 = GP[b01 ^ z0], G110 = GP[b11 ^ z0];
-    const float4 G001 = GP[b00 ^ z1], G101 = GP[b10 ^ z1], G011 = GP[b01 ^ z1], G111 = GP[b11 ^ z1];
+    const float4 G000 = uw_grad3(G3, b00 ^ z0), G100 = uw_grad3(G3, b10 ^ z0), G010 = uw_grad3(G3, b01 ^ z0),
+                 G110 = uw_grad3(G3, b11 ^ z0);
+    const float4 G001 = uw_grad3(G3, b00 ^ z1), G101 = uw_grad3(G3, b10 ^ z1), G011 = uw_grad3(G3, b01 ^ z1),
+                 G111 = uw_grad3(G3, b11 ^ z1);
     const float g000 = uw_gdot(G000, X.d, Y.d, Z.d), g100 = uw_gdot(G100, X.dm1, Y.d, Z.d);
     const float g010 = uw_gdot(G010, X.d, Y.dm1, Z.d), g110 = uw_gdot(G110, X.dm1, Y.dm1, Z.d);
     const float g001 = uw_gdot(G001, X.d, Y.d, Z.dm1), g101 = uw_gdot(G101, X.dm1, Y.d, Z.dm1);
@@ -85,7 +87,7 @@ struct BackArgs {
 
 template <int OMAX>
 __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
-    __shared__ float4 GP[256];
+    __shared__ float G3[3 * 256];
     __shared__ uint8_t P[256];
     __shared__ uw_axis AX[OMAX * kMaxL], AY[OMAX * kMaxL], AZ[OMAX * kMaxL];
     __shared__ float DF[3][OMAX * kMaxL];
@@ -111,7 +113,10 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     for (int i = tid; i < 256; i += kBlock) {
         const uint8_t p = a.tab->perm[i];
         P[i] = p;
-        GP[i] = a.tab->grad[p & 15];
+        const float4 g = a.tab->grad[p & 15];
+        G3[3 * i] = g.x;
+        G3[3 * i + 1] = g.y;
+        G3[3 * i + 2] = g.z;
     }
     __syncthreads();
     const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
@@ -162,7 +167,7 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
                 if (k < O) {
                     float g[3];
                     nk[k] = uw_octave_grad(AX[k * L + x], AY[k * L + y], AZ[k * L + z], DF[0][k * L + x],
-                                           DF[1][k * L + y], DF[2][k * L + z], P, GP, g);
+                                           DF[1][k * L + y], DF[2][k * L + z], P, G3, g);
                     const float af = a.f.amp[k] * float(a.f.octave_freq[k]);
                     total = fmaf(a.f.amp[k], nk[k], total);
                     gx = fmaf(af, g[0], gx);

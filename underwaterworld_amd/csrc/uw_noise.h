@@ -63,22 +63,30 @@ __device__ __forceinline__ float uw_gdot(float4 g, float x, float y, float z) {
 
 __device__ __forceinline__ float uw_lerp(float t, float a, float b) { return fmaf(t, b - a, a); }
 
+// Gradient of permutation slot j (the vector of code P[j] & 15) from a packed LDS table of three
+// floats per slot: with a 12-byte stride the slots of a gather spread over all LDS banks (a 16-byte
+// stride puts each component of every slot on the same 8 of the 32 banks).
+__device__ __forceinline__ float4 uw_grad3(const float* G3, uint32_t j) {
+    const float* g = G3 + 3u * j;
+    return make_float4(g[0], g[1], g[2], 0.0f);
+}
+
 // One octave of improved Perlin noise at the lattice cell / fractions given by X (hashed), Y, Z.
-// P: permutation table; GP[j]: gradient vector of code P[j] & 15.
+// P: permutation table; G3: gradient of each permutation slot (uw_grad3).
 __device__ __forceinline__ float uw_perlin_octave(const uw_axis& X, const uw_axis& Y, const uw_axis& Z,
-                                                  const uint8_t* P, const float4* GP) {
+                                                  const uint8_t* P, const float* G3) {
     const uint32_t a0 = X.bits & 255, a1 = X.bits >> 8;
     const uint32_t y0 = Y.bits & 255, y1 = Y.bits >> 8;
     const uint32_t z0 = Z.bits & 255, z1 = Z.bits >> 8;
     const uint32_t b00 = P[a0 ^ y0], b01 = P[a0 ^ y1], b10 = P[a1 ^ y0], b11 = P[a1 ^ y1];
-    const float g000 = uw_gdot(GP[b00 ^ z0], X.d, Y.d, Z.d);
-    const float g100 = uw_gdot(GP[b10 ^ z0], X.dm1, Y.d, Z.d);
-    const float g010 = uw_gdot(GP[b01 ^ z0], X.d, Y.dm1, Z.d);
-    const float g110 = uw_gdot(GP[b11 ^ z0], X.dm1, Y.dm1, Z.d);
-    const float g001 = uw_gdot(GP[b00 ^ z1], X.d, Y.d, Z.dm1);
-    const float g101 = uw_gdot(GP[b10 ^ z1], X.dm1, Y.d, Z.dm1);
-    const float g011 = uw_gdot(GP[b01 ^ z1], X.d, Y.dm1, Z.dm1);
-    const float g111 = uw_gdot(GP[b11 ^ z1], X.dm1, Y.dm1, Z.dm1);
+    const float g000 = uw_gdot(uw_grad3(G3, b00 ^ z0), X.d, Y.d, Z.d);
+    const float g100 = uw_gdot(uw_grad3(G3, b10 ^ z0), X.dm1, Y.d, Z.d);
+    const float g010 = uw_gdot(uw_grad3(G3, b01 ^ z0), X.d, Y.dm1, Z.d);
+    const float g110 = uw_gdot(uw_grad3(G3, b11 ^ z0), X.dm1, Y.dm1, Z.d);
+    const float g001 = uw_gdot(uw_grad3(G3, b00 ^ z1), X.d, Y.d, Z.dm1);
+    const float g101 = uw_gdot(uw_grad3(G3, b10 ^ z1), X.dm1, Y.d, Z.dm1);
+    const float g011 = uw_gdot(uw_grad3(G3, b01 ^ z1), X.d, Y.dm1, Z.dm1);
+    const float g111 = uw_gdot(uw_grad3(G3, b11 ^ z1), X.dm1, Y.dm1, Z.dm1);
     const float x00 = uw_lerp(X.fade, g000, g100);
     const float x10 = uw_lerp(X.fade, g010, g110);
     const float x01 = uw_lerp(X.fade, g001, g101);
@@ -127,14 +135,6 @@ struct uw_line_oct {
 __device__ __forceinline__ void uw_line_diff(uw_line_oct& s) {
     s.C = (s.A1 - s.B1) - s.A0;
     s.D = s.B1 - s.B0;
-}
-
-// Gradient of permutation slot j (the vector of code P[j] & 15) from a packed LDS table of three
-// floats per slot: with a 12-byte stride the slots of a gather spread over all LDS banks (a 16-byte
-// stride puts each component of every slot on the same 8 of the 32 banks).
-__device__ __forceinline__ float4 uw_grad3(const float* G3, uint32_t j) {
-    const float* g = G3 + 3u * j;
-    return make_float4(g[0], g[1], g[2], 0.0f);
 }
 
 // One z face of a line's lattice cell: A = lerp_y(lerp_x(P_..)), B = lerp_y(lerp_x(Gz_..)) for the

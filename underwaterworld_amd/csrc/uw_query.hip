@@ -14,12 +14,15 @@ constexpr int kBlock = 256;
 __global__ void __launch_bounds__(kBlock) k_density_at(uw_field f, const uw_tables* __restrict__ tab,
                                                        const double* __restrict__ xyz, int64_t npts,
                                                        float* __restrict__ out) {
-    __shared__ float4 GP[256];
+    __shared__ float G3[3 * 256];
     __shared__ uint8_t P[256];
     for (int i = threadIdx.x; i < 256; i += kBlock) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
-        GP[i] = tab->grad[p & 15];
+        const float4 g = tab->grad[p & 15];
+        G3[3 * i] = g.x;
+        G3[3 * i + 1] = g.y;
+        G3[3 * i + 2] = g.z;
     }
     __syncthreads();
     for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < npts; i += int64_t(gridDim.x) * kBlock) {
@@ -33,7 +36,7 @@ __global__ void __launch_bounds__(kBlock) k_density_at(uw_field f, const uw_tabl
             const uw_axis X = uw_hash_x(uw_axis_of(px, fk), P);
             const uw_axis Y = uw_axis_of(py, fk);
             const uw_axis Z = uw_axis_of(pz, fk);
-            total = fmaf(f.amp[k], uw_perlin_octave(X, Y, Z, P, GP), total);
+            total = fmaf(f.amp[k], uw_perlin_octave(X, Y, Z, P, G3), total);
         }
         float adj, post;
         uw_bias_terms(f, pz, &adj, &post);
