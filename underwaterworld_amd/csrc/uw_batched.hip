@@ -79,27 +79,41 @@ __global__ void k_reset(uint32_t* head, uw_totals* totals) {
 
 constexpr int kPrepThreads = 1024;
 constexpr int kOrderBatch = 8;  // chunks per thread whose loads are in flight together
-constexpr int kSurfBins = 8;                          // surface-distance classes per resolution
+constexpr int kSurfBins = 16;                         // surface-distance classes per resolution
 constexpr int kOrderBuckets = (kMaxN + 2) * kSurfBins;  // queue-order buckets
 static_assert(kOrderBuckets <= kPrepThreads, "one thread per bucket in the prefix");
 
-// Queue-order bucket of a chunk: resolution first (largest first, invalid last), then the distance
-// of the z-bias across the chunk from the isovalue (nearest first). The density is the bias plus
-// an fBm of magnitude at most 1, so a chunk whose bias range lies near the isovalue is the one
-// likely to hold surface, and to cost a mesh on top of its densities; chunks far from it are
-// blank or solid and cost the densities alone. Placing the costly ones first (longest task first)
-// leaves cheap tasks for the end of the launch, when the last round's balance sets the kernel
-// time. Only the processing order depends on it, never a result.
-__device__ __forceinline__ int order_bucket(const uw_field& f, int32_t n, int32_t cz) {
+// Queue-order bucket of a chunk: resolution first (largest first, invalid last), then how far an
+// estimate of the density at the chunk's centre lies from the isovalue (nearest first): the z-bias
+// plus the fBm's lowest octave (the largest amplitude), in single precision. A chunk whose centre
+// density is near the isovalue is the one likely to hold surface, and to cost a mesh on top of its
+// densities; chunks far from it are blank or solid and cost the densities alone. Placing the costly
+// ones first (longest task first) leaves cheap tasks for the end of the launch, when the last
+// round's balance sets the kernel time. Only the processing order depends on it, never a result.
+__device__ __forceinline__ uw_axis order_axis(float q) {
+    const float fl = floorf(q);
+    const int32_t l = int32_t(fl);
+    const float d = q - fl;
+    uw_axis a;
+    a.bits = uint32_t(l & 255) | (uint32_t((l + 1) & 255) << 8);
+    a.d = d;
+    a.dm1 = d - 1.0f;
+    a.fade = d * d * d * (d * (d * 6.0f - 15.0f) + 10.0f);
+    return a;
+}
+__device__ __forceinline__ int order_bucket(const uw_field& f, const uint8_t* P, const float* G3, int32_t n,
+                                            int32_t cx, int32_t cy, int32_t cz) {
     if (n < 1 || n > kMaxN) return (kMaxN + 1) * kSurfBins;
-    const float step = float(f.extent) / float(n);
-    float a0, p0, a1, p1;
-    uw_bias_terms(f, uw_sample_coord(f, 2, cz, 0, step), &a0, &p0);
-    uw_bias_terms(f, uw_sample_coord(f, 2, cz, n, step), &a1, &p1);
-    const float b0 = (a0 + p0) - f.iso, b1 = (a1 + p1) - f.iso;
-    // distance of [min(b0, b1), max(b0, b1)] from 0, in eighths
-    const float d = (b0 > 0.0f && b1 > 0.0f) ? fminf(b0, b1) : ((b0 < 0.0f && b1 < 0.0f) ? -fmaxf(b0, b1) : 0.0f);
-    const int k = d < 1.0f ? int(d * 8.0f) : kSurfBins - 1;
+    const float h = 0.5f * float(f.extent), fr = float(f.frequency), f0 = float(f.octave_freq[0]);
+    const float px = (float(f.extent * cx) + h + float(f.offset[0])) * fr;
+    const float py = (float(f.extent * cy) + h + float(f.offset[1])) * fr;
+    const float pz = (float(f.extent * cz) + h + float(f.offset[2])) * fr;
+    const uw_axis X = uw_hash_x(order_axis(px * f0), P), Y = order_axis(py * f0), Z = order_axis(pz * f0);
+    const float total = f.amp[0] * uw_perlin_octave(X, Y, Z, P, G3);
+    float adj, post;
+    uw_bias_terms(f, double(pz), &adj, &post);
+    const float d = fabsf(uw_density_from(total, f.amp_sum, adj, post) - f.iso);
+    const int k = d < 1.0f ? int(d * float(kSurfBins)) : kSurfBins - 1;
     return (kMaxN - n) * kSurfBins + (k < kSurfBins ? k : kSurfBins - 1);
 }
 
@@ -109,10 +123,11 @@ __device__ __forceinline__ int order_bucket(const uw_field& f, int32_t n, int32_
 // (resolution, largest first and invalid last, then nearest to the surface first). Chunks go in batches of kPrepThreads * kOrderBatch: each thread
 // issues all loads of its kOrderBatch chunks before using any, and a launch of at most one batch
 // keeps them in registers from the counting pass to the placing pass (one wait on memory).
-__device__ __forceinline__ void order_tasks(const uw_field& f, const int32_t* __restrict__ coords,
-                                            const int32_t* __restrict__ res, int32_t nchunks,
-                                            uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
-                                            uint32_t first_task, uint32_t* cnt, uint32_t* wsum) {
+__device__ __forceinline__ void order_tasks(const uw_field& f, const uw_tables* __restrict__ tab,
+                                            const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
+                                            int32_t nchunks, uw_task* __restrict__ tasks, uint32_t* head,
+                                            uw_totals* totals, uint32_t first_task, uint32_t* cnt, uint32_t* wsum,
+                                            uint8_t* P, float* G3) {
     constexpr int32_t kSpan = kPrepThreads * kOrderBatch;
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -124,6 +139,15 @@ __device__ __forceinline__ void order_tasks(const uw_field& f, const int32_t* __
         *totals = z;
     }
     if (tid < kOrderBuckets) cnt[tid] = 0;
+    if (tid < 256) {  // the noise tables, for the bucket estimate
+        const uint8_t p = tab->perm[tid];
+        P[tid] = p;
+        const float4 g = tab->grad[p & 15];
+        G3[3 * tid] = g.x;
+        G3[3 * tid + 1] = g.y;
+        G3[3 * tid + 2] = g.z;
+    }
+    __syncthreads();
     int32_t nb[kOrderBatch], xyz[kOrderBatch][3], bk[kOrderBatch];
     auto load = [&](int32_t base) {
 #pragma unroll
@@ -136,7 +160,7 @@ __device__ __forceinline__ void order_tasks(const uw_field& f, const int32_t* __
             xyz[j][2] = valid ? coords[3 * i + 2] : 0;
         }
 #pragma unroll
-        for (int j = 0; j < kOrderBatch; ++j) bk[j] = order_bucket(f, nb[j], xyz[j][2]);
+        for (int j = 0; j < kOrderBatch; ++j) bk[j] = order_bucket(f, P, G3, nb[j], xyz[j][0], xyz[j][1], xyz[j][2]);
     };
     load(0);
     __syncthreads();
@@ -181,15 +205,18 @@ __device__ __forceinline__ void order_tasks(const uw_field& f, const int32_t* __
 
 // Launch set-up in one kernel: workgroup 0 orders the queue (order_tasks); the others write the
 // chunks' z tables, one thread per record (uw_ztab_record).
-__global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const int32_t* __restrict__ coords,
+__global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const uw_tables* __restrict__ tab,
+                                                       const int32_t* __restrict__ coords,
                                                        const int32_t* __restrict__ res, int32_t nchunks,
                                                        uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
                                                        uint32_t first_task, int32_t stride,
                                                        unsigned char* __restrict__ ztab) {
     __shared__ uint32_t cnt[kOrderBuckets];
     __shared__ uint32_t wsum[kPrepThreads / 64];
+    __shared__ uint8_t P[256];
+    __shared__ float G3[3 * 256];
     if (blockIdx.x == 0) {
-        order_tasks(f, coords, res, nchunks, tasks, head, totals, first_task, cnt, wsum);
+        order_tasks(f, tab, coords, res, nchunks, tasks, head, totals, first_task, cnt, wsum, P, G3);
         return;
     }
     const uint64_t g = uint64_t(blockIdx.x - 1) * kPrepThreads + threadIdx.x;
@@ -488,7 +515,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
         }
         // chunk-level values are wave-uniform: keep them in scalar registers
         const int32_t c = __builtin_amdgcn_readfirstlane(s_next[0]);
-        if (c < 0) break;
+        if (c < 0 || c >= a.nchunks) break;  // end of the queue (the second test guards the info writes)
         const int32_t n = __builtin_amdgcn_readfirstlane(s_next[1]);
         const int32_t cx = __builtin_amdgcn_readfirstlane(s_next[2]);
         const int32_t cy = __builtin_amdgcn_readfirstlane(s_next[3]);
@@ -992,7 +1019,8 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.zstride = tune.max_res + 1;
     const uint64_t zblocks = (uint64_t(nchunks) * uint64_t(a.zstride + 1) + kPrepThreads - 1) / kPrepThreads;
     if (zblocks + 1 > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
-    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks + 1)), dim3(kPrepThreads), 0, st, a.f, coords, res, nchunks, tasks,
+    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks + 1)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords, res, nchunks,
+                       tasks,
                        head, a.totals, uint32_t(bl.grid), a.zstride, ztab);
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
