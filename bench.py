@@ -16,8 +16,9 @@ voxel samples/s of both, and the mixed-resolution workload (8192 chunks, 25 % ea
 F(O) = 78 O + 6 per sample (per launch F(4) * 13^3 * 4096 = 2.86 GFLOP), timed with HIP events
 recorded by the library on the launch stream around that kernel. `cpu_baseline`: the reference CPU
 path (the oracle's reference-faithful C restatement: f64 noise, literal linear-search dedup), one
-thread, over all 4096 chunks of the same workload; `cpu_baseline_all_cores` is the same build
-chunk-parallel over the host threads this process may use.
+thread, repeated passes over all 4096 chunks of the same workload for at least 10 s;
+`cpu_baseline_all_cores` is the same build chunk-parallel over the host threads this process may
+use, passes for at least 2 s.
 """
 from __future__ import annotations
 
@@ -280,18 +281,26 @@ def main():
     cpu = cpu_all = None
     if not args.no_cpu:
         from oracle import oracle as O
-        t0 = time.perf_counter()
-        O.build_chunks(p, coords, n, nthreads=1, dedup="linear")
-        dt = time.perf_counter() - t0
-        cpu = {"value": round(nch / dt, 2), "unit": "chunks/s", "cores": 1, "kind": "port",
-               "sample": f"all {nch} chunks of the workload, reference-faithful C restatement "
-                         "(f64 noise, linear-search dedup), 1 thread", "seconds": round(dt, 3)}
+
+        def timed_passes(nthreads: int, min_s: float, max_passes: int = 40):
+            """Whole-workload passes until `min_s` seconds have elapsed (bounded)."""
+            passes, t0 = 0, time.perf_counter()
+            while True:
+                O.build_chunks(p, coords, n, nthreads=nthreads, dedup="linear")
+                passes += 1
+                dt = time.perf_counter() - t0
+                if dt >= min_s or passes >= max_passes:
+                    return passes, dt
+
+        k, dt = timed_passes(1, 10.0)
+        cpu = {"value": round(k * nch / dt, 2), "unit": "chunks/s", "cores": 1, "kind": "port",
+               "sample": f"{k} passes over all {nch} chunks of the workload ({k * nch} chunk builds), "
+                         "reference-faithful C restatement (f64 noise, linear-search dedup), 1 thread",
+               "seconds": round(dt, 3)}
         nthr = max(1, min(len(os.sched_getaffinity(0)), int(os.environ.get("OMP_NUM_THREADS", "16") or 16), 64))
-        t0 = time.perf_counter()
-        O.build_chunks(p, coords, n, nthreads=nthr, dedup="linear")
-        dt = time.perf_counter() - t0
-        cpu_all = {"value": round(nch / dt, 2), "unit": "chunks/s", "cores": nthr, "kind": "port",
-                   "sample": f"all {nch} chunks, same build, chunk-parallel over {nthr} threads",
+        k, dt = timed_passes(nthr, 2.0)
+        cpu_all = {"value": round(k * nch / dt, 2), "unit": "chunks/s", "cores": nthr, "kind": "port",
+                   "sample": f"{k} passes over all {nch} chunks, same build, chunk-parallel over {nthr} threads",
                    "seconds": round(dt, 3)}
 
     # headline: the faster of the two batched launch forms (both are reported)
