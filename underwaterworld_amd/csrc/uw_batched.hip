@@ -36,7 +36,7 @@ constexpr size_t kQueueBytes = 512;
 constexpr int kResvIndsOffset = 16;  // u64 words from the vertex counter to the index counter
 
 // One queued chunk, in queue order (32 bytes, two 16-byte loads): chunk index c (-1: queue
-// empty), resolution n and chunk coordinates. Written by k_prep, read once per task.
+// empty), resolution n and chunk coordinates. Written by k_place, read once per task.
 struct __align__(16) uw_task {
     int32_t c, n, cx, cy;
     int32_t cz, pad[3];
@@ -77,11 +77,8 @@ __global__ void k_reset(uint32_t* head, uw_totals* totals) {
     }
 }
 
-constexpr int kPrepThreads = 1024;
-constexpr int kOrderBatch = 8;  // chunks per thread whose loads are in flight together
 constexpr int kSurfBins = 16;                         // surface-distance classes per resolution
 constexpr int kOrderBuckets = (kMaxN + 2) * kSurfBins;  // queue-order buckets
-static_assert(kOrderBuckets <= kPrepThreads, "one thread per bucket in the prefix");
 
 // Queue-order bucket of a chunk: resolution first (largest first, invalid last), then how far an
 // estimate of the density at the chunk's centre lies from the isovalue (nearest first): the z-bias
@@ -117,20 +114,97 @@ __device__ __forceinline__ int order_bucket(const uw_field& f, const uint8_t* P,
     return (kMaxN - n) * kSurfBins + (k < kSurfBins ? k : kSurfBins - 1);
 }
 
-// Queue set-up (one workgroup of kPrepThreads): resets the output totals, the range counters and
-// the queue head (to first_task: the persistent kernel's workgroup g starts with task g without
-// claiming it), and writes the task records in queue order, a counting sort by order_bucket
-// (resolution, largest first and invalid last, then nearest to the surface first). Chunks go in batches of kPrepThreads * kOrderBatch: each thread
-// issues all loads of its kOrderBatch chunks before using any, and a launch of at most one batch
-// keeps them in registers from the counting pass to the placing pass (one wait on memory).
-__device__ __forceinline__ void order_tasks(const uw_field& f, const uw_tables* __restrict__ tab,
-                                            const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
-                                            int32_t nchunks, uw_task* __restrict__ tasks, uint32_t* head,
-                                            uw_totals* totals, uint32_t first_task, uint32_t* cnt, uint32_t* wsum,
-                                            uint8_t* P, float* G3) {
-    constexpr int32_t kSpan = kPrepThreads * kOrderBatch;
+// Queue set-up, a counting sort by order_bucket (resolution, largest first and invalid last, then
+// nearest to the surface first) in two launches that each spread over the device (one workgroup
+// would spend ~5 us on the bucket estimates of 4096 chunks alone):
+//  k_prep:  workgroups [0, nest) take kPrepThreads chunks each: bucket estimate, rank within the
+//           bucket among the workgroup's chunks (LDS atomic), the workgroup's bucket histogram;
+//           the other workgroups write the chunks' z tables, one thread per record.
+//  k_place: workgroup j places the chunks of estimate workgroup j: bucket offsets from all
+//           histograms (total over all workgroups, plus the part of workgroups before j), then
+//           slot = offset + rank. Workgroup 0 also resets the totals, the range counters and the
+//           queue head (to first_task: the persistent kernel's workgroup g starts with task g
+//           without claiming it).
+// The histograms are written whole (zeros included) by every launch, so no state carries over.
+// Per-chunk key: bucket | rank << 16 (rank < kPrepThreads).
+constexpr int kPrepThreads = 256;
+static_assert(kOrderBuckets < 65536 && kPrepThreads <= 65536, "bucket and rank share a 32-bit key");
+
+struct OrderBufs {
+    size_t keys_off, hist_off, bytes;  // byte offsets after the task records, total bytes
+};
+__host__ __device__ inline int32_t order_blocks(int32_t nchunks) { return (nchunks + kPrepThreads - 1) / kPrepThreads; }
+inline OrderBufs order_bufs(int32_t nchunks) {
+    OrderBufs o;
+    o.keys_off = size_t(nchunks) * sizeof(uw_task);
+    o.hist_off = (o.keys_off + size_t(nchunks) * sizeof(uint32_t) + 255) & ~size_t(255);
+    o.bytes = o.hist_off + size_t(order_blocks(nchunks)) * kOrderBuckets * sizeof(uint32_t);
+    return o;
+}
+
+__device__ __forceinline__ void estimate_block(const uw_field& f, const uw_tables* __restrict__ tab,
+                                               const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
+                                               int32_t nchunks, uint32_t* __restrict__ keys,
+                                               uint32_t* __restrict__ hist) {
+    __shared__ uint32_t cnt[kOrderBuckets];
+    __shared__ uint8_t P[256];
+    __shared__ float G3[3 * 256];
+    __shared__ float4 G16[16];
     const int tid = threadIdx.x;
-    if (tid == 0) {
+    const int32_t c = int32_t(blockIdx.x) * kPrepThreads + tid;
+    const bool valid = c < nchunks;
+    // the noise tables and the chunk, loaded side by side
+    const uint32_t perm = tab->perm[tid];
+    float4 g16 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tid < 16) g16 = tab->grad[tid];
+    const int32_t n = valid ? res[c] : 0;
+    const int32_t cx = valid ? coords[3 * c] : 0, cy = valid ? coords[3 * c + 1] : 0,
+                  cz = valid ? coords[3 * c + 2] : 0;
+    for (int k = tid; k < kOrderBuckets; k += kPrepThreads) cnt[k] = 0u;
+    if (tid < 16) G16[tid] = g16;
+    __syncthreads();
+    P[tid] = uint8_t(perm);
+    {  // slot j holds the gradient of code P[j] & 15, as the density loops' tables do
+        const float4 g = G16[perm & 15];
+        G3[3 * tid] = g.x;
+        G3[3 * tid + 1] = g.y;
+        G3[3 * tid + 2] = g.z;
+    }
+    __syncthreads();
+    const int bk = order_bucket(f, P, G3, n, cx, cy, cz);
+    const uint32_t rank = valid ? atomicAdd(&cnt[bk], 1u) : 0u;
+    __syncthreads();
+    uint32_t* h = hist + size_t(blockIdx.x) * kOrderBuckets;
+    for (int k = tid; k < kOrderBuckets; k += kPrepThreads) h[k] = cnt[k];
+    if (valid) keys[c] = uint32_t(bk) | (rank << 16);
+}
+
+static_assert(kPrepThreads == 256 && kPrepThreads * 3 >= kOrderBuckets, "three buckets per thread in k_place");
+
+__global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const uw_tables* __restrict__ tab,
+                                                       const int32_t* __restrict__ coords,
+                                                       const int32_t* __restrict__ res, int32_t nchunks,
+                                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ hist,
+                                                       int32_t nest, int32_t stride, unsigned char* __restrict__ ztab) {
+    if (int32_t(blockIdx.x) < nest) {
+        estimate_block(f, tab, coords, res, nchunks, keys, hist);
+        return;
+    }
+    const uint64_t g = uint64_t(blockIdx.x - uint32_t(nest)) * kPrepThreads + threadIdx.x;
+    uw_ztab_record(f, coords, res, 0, nchunks, stride, ztab, g);
+}
+
+__global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restrict__ coords,
+                                                        const int32_t* __restrict__ res, int32_t nchunks,
+                                                        const uint32_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ hist, int32_t nest,
+                                                        uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
+                                                        uint32_t first_task) {
+    __shared__ uint32_t base[3 * kPrepThreads];
+    __shared__ uint32_t wsum[kPrepThreads / 64];
+    const int tid = threadIdx.x;
+    const int32_t j = int32_t(blockIdx.x);
+    if (j == 0 && tid == 0) {
         *head = first_task;
         unsigned long long* resv = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned char*>(head) + 128);
         resv[0] = 0ull;
@@ -138,89 +212,58 @@ __device__ __forceinline__ void order_tasks(const uw_field& f, const uw_tables* 
         uw_totals z = {0, 0, 0, 0};
         *totals = z;
     }
-    if (tid < kOrderBuckets) cnt[tid] = 0;
-    if (tid < 256) {  // the noise tables, for the bucket estimate
-        const uint8_t p = tab->perm[tid];
-        P[tid] = p;
-        const float4 g = tab->grad[p & 15];
-        G3[3 * tid] = g.x;
-        G3[3 * tid + 1] = g.y;
-        G3[3 * tid + 2] = g.z;
-    }
-    __syncthreads();
-    int32_t nb[kOrderBatch], xyz[kOrderBatch][3], bk[kOrderBatch];
-    auto load = [&](int32_t base) {
+    const int32_t c = j * kPrepThreads + tid;
+    const bool valid = c < nchunks;
+    const uint32_t key = valid ? keys[c] : 0u;
+    const int32_t n = valid ? res[c] : 0;
+    const int32_t cx = valid ? coords[3 * c] : 0, cy = valid ? coords[3 * c + 1] : 0,
+                  cz = valid ? coords[3 * c + 2] : 0;
+    // buckets 3 tid .. 3 tid + 2: totals over all estimate workgroups, and over those before j
+    // (kHistBatch workgroups' loads in flight together: a loop that waits per workgroup costs one
+    // L2 round trip each)
+    constexpr int kHistBatch = 8;
+    uint32_t tot[3] = {0u, 0u, 0u}, before[3] = {0u, 0u, 0u};
+    for (int32_t w0 = 0; w0 < nest; w0 += kHistBatch) {
+        uint32_t v[kHistBatch][3];
 #pragma unroll
-        for (int j = 0; j < kOrderBatch; ++j) {
-            const int32_t i = base + j * kPrepThreads + tid;
-            const bool valid = i < nchunks;
-            nb[j] = valid ? res[i] : 0;
-            xyz[j][0] = valid ? coords[3 * i] : 0;
-            xyz[j][1] = valid ? coords[3 * i + 1] : 0;
-            xyz[j][2] = valid ? coords[3 * i + 2] : 0;
-        }
+        for (int u = 0; u < kHistBatch; ++u) {
+            const int32_t w = w0 + u;
+            const uint32_t* h = hist + size_t(w) * kOrderBuckets;
 #pragma unroll
-        for (int j = 0; j < kOrderBatch; ++j) bk[j] = order_bucket(f, P, G3, nb[j], xyz[j][0], xyz[j][1], xyz[j][2]);
-    };
-    load(0);
-    __syncthreads();
-    // bucket counts and, below, slots: one LDS atomic per chunk (lanes of one bucket serialise in
-    // the LDS, and a wave holds chunks of many buckets)
-    for (int32_t base = 0; base < nchunks; base += kSpan) {
-        if (base > 0) load(base);
-#pragma unroll
-        for (int j = 0; j < kOrderBatch; ++j)
-            if (base + j * kPrepThreads + tid < nchunks) atomicAdd(&cnt[bk[j]], 1u);
-    }
-    __syncthreads();
-    {  // exclusive prefix over the buckets: one bucket per thread, wave scans, then wave offsets
-        const uint32_t v = tid < kOrderBuckets ? cnt[tid] : 0u;
-        const uint32_t inc = uw_wave_incl_scan(v);
-        if ((tid & 63) == 63) wsum[tid >> 6] = inc;
-        __syncthreads();
-        uint32_t off = 0;
-        for (int w = 0; w < (tid >> 6); ++w) off += wsum[w];
-        if (tid < kOrderBuckets) cnt[tid] = off + inc - v;
-    }
-    __syncthreads();
-    for (int32_t base = 0; base < nchunks; base += kSpan) {
-        if (nchunks > kSpan) load(base);
-#pragma unroll
-        for (int j = 0; j < kOrderBatch; ++j) {
-            const int32_t i = base + j * kPrepThreads + tid;
-            if (i < nchunks) {
-                const uint32_t slot = atomicAdd(&cnt[bk[j]], 1u);
-                uw_task t;
-                t.c = i;
-                t.n = nb[j];
-                t.cx = xyz[j][0];
-                t.cy = xyz[j][1];
-                t.cz = xyz[j][2];
-                t.pad[0] = t.pad[1] = t.pad[2] = 0;
-                tasks[slot] = t;
+            for (int k = 0; k < 3; ++k) {
+                const int b = 3 * tid + k;
+                v[u][k] = (w < nest && b < kOrderBuckets) ? h[b] : 0u;
             }
         }
+#pragma unroll
+        for (int u = 0; u < kHistBatch; ++u)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                tot[k] += v[u][k];
+                before[k] += w0 + u < j ? v[u][k] : 0u;
+            }
     }
-}
-
-// Launch set-up in one kernel: workgroup 0 orders the queue (order_tasks); the others write the
-// chunks' z tables, one thread per record (uw_ztab_record).
-__global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const uw_tables* __restrict__ tab,
-                                                       const int32_t* __restrict__ coords,
-                                                       const int32_t* __restrict__ res, int32_t nchunks,
-                                                       uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
-                                                       uint32_t first_task, int32_t stride,
-                                                       unsigned char* __restrict__ ztab) {
-    __shared__ uint32_t cnt[kOrderBuckets];
-    __shared__ uint32_t wsum[kPrepThreads / 64];
-    __shared__ uint8_t P[256];
-    __shared__ float G3[3 * 256];
-    if (blockIdx.x == 0) {
-        order_tasks(f, tab, coords, res, nchunks, tasks, head, totals, first_task, cnt, wsum, P, G3);
-        return;
+    // exclusive prefix of the bucket totals: three per thread, wave scans, then wave offsets
+    const uint32_t mine = tot[0] + tot[1] + tot[2];
+    const uint32_t inc = uw_wave_incl_scan(mine);
+    if ((tid & 63) == 63) wsum[tid >> 6] = inc;
+    __syncthreads();
+    uint32_t off = inc - mine;
+    for (int w = 0; w < (tid >> 6); ++w) off += wsum[w];
+    base[3 * tid] = off + before[0];
+    base[3 * tid + 1] = off + tot[0] + before[1];
+    base[3 * tid + 2] = off + tot[0] + tot[1] + before[2];
+    __syncthreads();
+    if (valid) {
+        uw_task t;
+        t.c = c;
+        t.n = n;
+        t.cx = cx;
+        t.cy = cy;
+        t.cz = cz;
+        t.pad[0] = t.pad[1] = t.pad[2] = 0;
+        tasks[base[key & 0xFFFFu] + (key >> 16)] = t;
     }
-    const uint64_t g = uint64_t(blockIdx.x - 1) * kPrepThreads + threadIdx.x;
-    uw_ztab_record(f, coords, res, 0, nchunks, stride, ztab, g);
 }
 
 // Reserves a chunk's contiguous vertex and index ranges (one thread). A packed launch (worst-case
@@ -937,7 +980,7 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         (void)hipFree(ctx->d_batch_order);
         ctx->d_batch_order = nullptr;
         ctx->batch_order_cap = 0;
-        UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, size_t(nchunks) * sizeof(uw_task)));
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, order_bufs(nchunks).bytes));
         ctx->batch_order_cap = size_t(nchunks);
     }
     if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, kQueueBytes));
@@ -1018,10 +1061,15 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.dscr_stride = bl.dscr_stride;
     a.zstride = tune.max_res + 1;
     const uint64_t zblocks = (uint64_t(nchunks) * uint64_t(a.zstride + 1) + kPrepThreads - 1) / kPrepThreads;
-    if (zblocks + 1 > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
-    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks + 1)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords, res, nchunks,
-                       tasks,
-                       head, a.totals, uint32_t(bl.grid), a.zstride, ztab);
+    const int32_t nest = order_blocks(nchunks);
+    if (zblocks + uint64_t(nest) > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
+    const OrderBufs ob = order_bufs(nchunks);
+    uint32_t* keys = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(task_buf) + ob.keys_off);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(task_buf) + ob.hist_off);
+    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks) + uint32_t(nest)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords,
+                       res, nchunks, keys, hist, nest, a.zstride, ztab);
+    hipLaunchKernelGGL(k_place, dim3(uint32_t(nest)), dim3(kPrepThreads), 0, st, coords, res, nchunks, keys, hist, nest,
+                       tasks, head, a.totals, uint32_t(bl.grid));
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
@@ -1097,9 +1145,10 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     BatchLaunch bl;
     if (int rc = prepare_batched(ctx, nchunks, &bl)) return rc;
     // the graph owns its queue state, task records and z tables (the context's may be reallocated):
-    // queue state at byte 0, records after it, z tables after them
+    // queue state at byte 0, task records (with the ordering keys and histograms) after it, z tables
+    // after them
     uint32_t* gbuf = nullptr;
-    const size_t ord_bytes = (kQueueBytes + size_t(nchunks) * sizeof(uw_task) + 255) & ~size_t(255);
+    const size_t ord_bytes = (kQueueBytes + order_bufs(nchunks).bytes + 255) & ~size_t(255);
     const size_t ztab_bytes = (bl.ztab_bytes + 255) & ~size_t(255);
     UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + ztab_bytes + bl.dscr_bytes));
     unsigned char* gtasks = reinterpret_cast<unsigned char*>(gbuf) + kQueueBytes;
