@@ -8,7 +8,7 @@ is written to preallocated HBM buffers.
 
 Paths: "plain" = one launch per builder stage over all chunks (density, classify + compact,
 offsets, emit); "batched" = one persistent launch (plus two set-up kernels that write the z tables and
-order the queue, and a one-workgroup totals kernel), also replayed from a captured hipGraph. `value` = chunks/s of the batched path. Extra fields: plain chunks/s,
+order the queue), also replayed from a captured hipGraph. `value` = chunks/s of the batched path. Extra fields: plain chunks/s,
 voxel samples/s of both, and the mixed-resolution workload (8192 chunks, 25 % each at 8^3, 12^3,
 24^3, 32^3, 6 octaves) through the batched path.
 

@@ -30,10 +30,12 @@ constexpr int kRing = 2048;       // cell ring (vertex base << 8 | case), power 
 constexpr int kMinList = 512;     // smallest compacted active-cell list
 constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
 // Queue state (one allocation, 512 bytes): the task head (u32) at byte 0, the vertex range counter
-// (u64; in packed launches vertices | indices << 32) at byte 128 and the index range counter (u64,
-// unpacked launches) at byte 256, each in its own cache line.
+// (u64; in packed launches vertices | indices << 32) at byte 128, the index range counter (u64,
+// unpacked launches) at byte 256 and the exit counter (u64: workgroups done | non-blank chunks << 32)
+// at byte 384, each in its own cache line.
 constexpr size_t kQueueBytes = 512;
 constexpr int kResvIndsOffset = 16;  // u64 words from the vertex counter to the index counter
+constexpr int kExitOffset = 32;      // u64 words from the vertex counter to the exit counter
 
 // One queued chunk, in queue order (32 bytes, two 16-byte loads): chunk index c (-1: queue
 // empty), resolution n and chunk coordinates. Written by k_place, read once per task.
@@ -209,6 +211,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restric
         unsigned long long* resv = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned char*>(head) + 128);
         resv[0] = 0ull;
         resv[kResvIndsOffset] = 0ull;
+        resv[kExitOffset] = 0ull;
         uw_totals z = {0, 0, 0, 0};
         *totals = z;
     }
@@ -269,7 +272,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restric
 // Reserves a chunk's contiguous vertex and index ranges (one thread). A packed launch (worst-case
 // totals below 2^32) takes both with one 64-bit atomic on vertices | indices << 32, halving the
 // traffic on the one contended counter; otherwise one atomic per range. The launch totals are
-// written after the kernel (k_finalize), so no other atomic goes to a shared counter per chunk.
+// written by the last workgroup to leave, so no other atomic goes to a shared counter per chunk.
 __device__ __forceinline__ void reserve_ranges(const BatchArgs& a, uint32_t nv, uint32_t ni, unsigned long long* vo,
                                                unsigned long long* io, uint32_t* flags) {
     if (a.packed) {
@@ -283,39 +286,6 @@ __device__ __forceinline__ void reserve_ranges(const BatchArgs& a, uint32_t nv, 
     if (*vo + nv > a.vcap || *io + ni > a.icap) {
         *flags |= UW_CHUNK_OVERFLOW;
         atomicOr(&a.totals->overflow, 1u);
-    }
-}
-
-// Launch totals, after the persistent kernel (one workgroup): the range counters hold the vertex
-// and index totals, and the non-blank chunks are those with indices. Writing them here keeps the
-// persistent kernel free of per-chunk or per-workgroup atomics on the totals, which, issued by
-// every workgroup as it finishes, queue on one counter at the end of the launch.
-__global__ void __launch_bounds__(1024) k_finalize(const unsigned long long* __restrict__ resv, int32_t packed,
-                                                   const uw_chunk_info* __restrict__ info, int32_t nchunks,
-                                                   uw_totals* totals) {
-    __shared__ uint32_t s_cnt;
-    if (threadIdx.x == 0) s_cnt = 0u;
-    __syncthreads();
-    // loads of up to 8 records per thread in flight together
-    uint32_t cnt = 0;
-    for (int32_t base = 0; base < nchunks; base += 8 * 1024) {
-        uint32_t ni[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int32_t i = base + j * 1024 + int32_t(threadIdx.x);
-            ni[j] = i < nchunks ? info[i].n_inds : 0u;
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) cnt += ni[j] > 0u ? 1u : 0u;
-    }
-    cnt = uw_wave_last(uw_wave_incl_scan(cnt));
-    if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&s_cnt, cnt);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long r0 = resv[0];
-        totals->n_verts = packed ? (r0 & 0xFFFFFFFFull) : r0;
-        totals->n_inds = packed ? (r0 >> 32) : resv[kResvIndsOffset];
-        totals->n_nonblank = s_cnt;
     }
 }
 
@@ -548,6 +518,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     // claimed when the block is free, which keeps the end of the launch balanced.
     const bool pf = tid == BLOCK - 1;
     if (pf) stage_task(a, blockIdx.x, s_next);
+    uint32_t nonblank = 0;  // thread 0: chunks of this workgroup with indices
     bool staged = true;  // block-uniform: s_next holds the next task
 
     for (;;) {
@@ -662,6 +633,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 uint32_t flags = ni == 0 ? uint32_t(UW_CHUNK_BLANK) : 0u;
                 unsigned long long vo = 0, io = 0;
                 if (ni > 0) reserve_ranges(a, nv, ni, &vo, &io, &flags);
+                nonblank += ni > 0 ? 1u : 0u;
                 if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
                 uw_chunk_info r = {flags, nv, ni, na, vo, io};
                 a.info[c] = r;
@@ -804,6 +776,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
             if (!below) flags = UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP;
             else if (ni == 0) flags = UW_CHUNK_BLANK;
             if (ni > 0) reserve_ranges(a, nv, ni, &vo, &io, &flags);
+            nonblank += ni > 0 ? 1u : 0u;
             if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
             uw_chunk_info r = {flags, nv, ni, na, vo, io};
             a.info[c] = r;
@@ -881,6 +854,20 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                     pending_rounds = 0;
                 }
             }
+        }
+    }
+    // Launch totals, written by the last workgroup to leave: one 64-bit exit atomic per workgroup
+    // (done | non-blank chunks << 32) on its own line, spread over the end of the launch; the
+    // reservation atomics of every workgroup returned before its exit atomic, so the range
+    // counters read here (atomically, at the same coherence point) are final.
+    if (tid == 0) {
+        unsigned long long* exitc = a.resv + kExitOffset;
+        const unsigned long long old = atomicAdd(exitc, 1ull | (static_cast<unsigned long long>(nonblank) << 32));
+        if (uint32_t(old) == gridDim.x - 1u) {
+            const unsigned long long r0 = atomicAdd(a.resv, 0ull);
+            a.totals->n_verts = a.packed ? (r0 & 0xFFFFFFFFull) : r0;
+            a.totals->n_inds = a.packed ? (r0 >> 32) : atomicAdd(a.resv + kResvIndsOffset, 0ull);
+            a.totals->n_nonblank = uint32_t(old >> 32) + nonblank;
         }
     }
 }
@@ -1073,7 +1060,6 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, st, a.resv, a.packed, a.info, nchunks, a.totals);
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
 }
