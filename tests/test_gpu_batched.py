@@ -85,6 +85,23 @@ def test_batched_unpacked_reservations():
     _assert_totals(b, ref)
 
 
+def test_batched_queue_setup_forms():
+    # Queue set-up at small and large launch sizes (20000 chunks: 79 ordering workgroups, more than
+    # one batch of histogram loads in the placement kernel), alternating on one context so nothing
+    # of a launch's set-up state leaks into the next; each equal to the plain path chunk by chunk.
+    from underwaterworld_amd import Perlin, build_chunks_batched
+    perlin = Perlin(PR.preset("chunks12"))
+    big = [(x, y, z) for z in (-1, 0) for y in range(-50, 50) for x in range(-50, 50)]
+    assert len(big) == 20000 and (len(big) + 255) // 256 > 8
+    small = PR.grid_coords(-4, 4)
+    for coords, n in ((small, 12), (big, 6), (small, 12), (big[:300], 6)):
+        ref = _plain_by_resolution(perlin, coords, [n] * len(coords))
+        b = build_chunks_batched(perlin, coords, [n] * len(coords))
+        _sync()
+        _assert_same(b, ref, len(coords))
+        _assert_totals(b, ref)
+
+
 def test_batched_equals_plain_full_grid_12():
     from underwaterworld_amd import Perlin, build_chunks_batched
     p = PR.preset("chunks12")
