@@ -222,9 +222,9 @@ __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restric
     const int32_t cx = valid ? coords[3 * c] : 0, cy = valid ? coords[3 * c + 1] : 0,
                   cz = valid ? coords[3 * c + 2] : 0;
     // buckets 3 tid .. 3 tid + 2: totals over all estimate workgroups, and over those before j
-    // (kHistBatch workgroups' loads in flight together: a loop that waits per workgroup costs one
-    // L2 round trip each)
-    constexpr int kHistBatch = 8;
+    // (kHistBatch workgroups' loads in flight together, one L2 round trip for launches of up to
+    // 4096 chunks: a loop that waits per workgroup costs one round trip each)
+    constexpr int kHistBatch = 16;
     uint32_t tot[3] = {0u, 0u, 0u}, before[3] = {0u, 0u, 0u};
     for (int32_t w0 = 0; w0 < nest; w0 += kHistBatch) {
         uint32_t v[kHistBatch][3];
