@@ -26,7 +26,11 @@
 namespace {
 
 constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
-constexpr int kRing = 2048;       // cell ring (vertex base << 8 | case), power of two
+// x-slab cell ring (vertex base << 8 | case), a power of two: the larger ring (fewer, fuller
+// emission flushes) when it costs no resident workgroups, or when the launch's chunks still get a
+// workgroup each; else the smaller
+constexpr int kRingMax = 4096, kRingMin = 2048;
+constexpr int kOnePassMax = 2048; // most cells of a chunk built in one pass (the compacted list's cap)
 constexpr int kMinList = 512;     // smallest compacted active-cell list
 constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
 // Queue state (one allocation, 512 bytes): the task head (u32) at byte 0, the vertex range counter
@@ -65,7 +69,7 @@ struct BatchArgs {
     uint64_t vcap, icap;
     int32_t max_res;      // largest resolution accepted by this launch (sizes the LDS)
     int32_t list_cap;     // entries of the compacted active-cell list
-    int32_t ring_cap;     // entries of the cell ring: kRing, or list_cap when every chunk is one-pass
+    int32_t ring_cap;     // entries of the cell ring: kRingMax or kRingMin (a power of two), or list_cap when every chunk is one-pass
     int32_t has_lioff;    // 1: a separate index-offset list follows the active list (x-slab chunks)
     int32_t axis_stride;  // entries per octave row of the x / y tables: max_res + 1
     int8_t sx[kMaxN + 1]; // cell planes per slab for each resolution
@@ -789,7 +793,8 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
 
         const unsigned long long vo = s_vo, io = s_io;
         // rounds that may stay pending before the cell ring would overwrite an owner
-        const int32_t max_pending_rounds = max(1, (kRing - nn - 1) / BLOCK - 1);
+        const int32_t ring = a.ring_cap;  // x-slab launches: a power of two
+        const int32_t max_pending_rounds = max(1, (ring - nn - 1) / BLOCK - 1);
         const uint32_t list_limit = uint32_t(a.list_cap - BLOCK);
         auto owner_voff = [](uint32_t, uint32_t qw) -> uint32_t { return qw >> 8; };
         uint32_t carry_v = 0, carry_i = 0;
@@ -825,7 +830,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                 unsigned long long rt;
                 const unsigned long long pre = block_scan_u64<BLOCK>(packed, &rt, scan_lds);
                 if (g < gend) {
-                    RING[g & (kRing - 1)] = ((carry_v + uint32_t(pre & 0xFFFFull)) << 8) | cc.cs;
+                    RING[g & (ring - 1)] = ((carry_v + uint32_t(pre & 0xFFFFull)) << 8) | cc.cs;
                     if (cc.act) {
                         const uint32_t slot = list_n + uint32_t(pre >> 32);
                         LIST[slot] = uint32_t(g);
@@ -841,13 +846,13 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
                     __syncthreads();
                     for (uint32_t e = tid; e < list_n; e += BLOCK) {
                         const uint32_t gg = LIST[e];
-                        const uint32_t w = RING[gg & (kRing - 1)];
+                        const uint32_t w = RING[gg & (ring - 1)];
                         int32_t x, y, z;
                         uw_cell_xyz(gg, dn, dnn, &x, &y, &z);
                         float r[8];
                         cell_corners(DENS, dns, L, x, y, z, r);
                         emit_cell(a, n, L, x, y, z, w & 255u, r, w >> 8, vo, io + LIOFF[e], step, ox, oy, oz, TRI,
-                                  LUT, BEF, RING, uint32_t(kRing - 1), owner_voff, vo + nv, io + ni);
+                                  LUT, BEF, RING, uint32_t(ring - 1), owner_voff, vo + nv, io + ni);
                     }
                     __syncthreads();
                     list_n = 0;
@@ -895,21 +900,21 @@ BatchedKernel pick_kernel(int block, int octaves) {
 }
 
 // True when every resolution up to max_res is built in one pass (one slab, n^3 <= list_cap):
-// the launch then needs no x-slab buffers (a kRing cell ring and the separate offset list).
+// the launch then needs no x-slab buffers (a cell ring and the separate offset list).
 bool one_pass_only(const uw_batch_tuning& t, int32_t list_cap) {
     for (int n = 1; n <= t.max_res; ++n)
         if (t.slab[n] != n || n * n * n > list_cap) return false;
     return true;
 }
 
-// Compacted-list capacity: every chunk built in one pass (one slab, n^3 <= kRing cells) must fit,
+// Compacted-list capacity: every chunk built in one pass (one slab, n^3 <= kOnePassMax cells) must fit,
 // and a round of `block` cells must fit on top of a flushable list.
 int32_t list_cap_for(const uw_batch_tuning& t) {
     int32_t cap = kMinList > t.block ? kMinList : t.block;
     for (int n = 1; n <= t.max_res; ++n)
-        if (t.slab[n] == n && n * n * n <= kRing) cap = n * n * n > cap ? n * n * n : cap;
+        if (t.slab[n] == n && n * n * n <= kOnePassMax) cap = n * n * n > cap ? n * n * n : cap;
     cap = (cap + 3) & ~3;  // keeps the LDS tables after the list 16-byte aligned
-    return cap > kRing ? kRing : cap;
+    return cap > kOnePassMax ? kOnePassMax : cap;
 }
 
 size_t dens_bytes_for(const uw_batch_tuning& t) {
@@ -955,6 +960,7 @@ struct BatchLaunch {
     BatchedKernel k;
     size_t lds;
     int grid;
+    int32_t ring;       // cell ring entries (list_cap when every chunk is one-pass)
     size_t ztab_bytes;  // z tables of nchunks chunks
     uint64_t dscr_stride;  // density scratch floats per workgroup (0: no resolution of the shape has x-slabs)
     size_t dscr_bytes;     // density scratch of the whole grid
@@ -982,18 +988,34 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
     bl->k = pick_kernel(tune.block, octaves);
     const int32_t list_cap = list_cap_for(tune);
     const bool one_pass = one_pass_only(tune, list_cap);
-    bl->lds = lds_fixed_bytes(size_t(list_cap), size_t(one_pass ? list_cap : kRing), one_pass ? 0 : 1) +
-              lds_table_bytes(octaves, tune.max_res + 1) + dens_bytes_for(tune);
-    UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(bl->k), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    int(bl->lds)));
-    bl->grid = tune.grid;
-    if (bl->grid == 0) {
-        int per_cu = 0;
-        UW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bl->k),
-                                                                 tune.block, bl->lds));
-        if (per_cu < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "batched kernel does not fit on a CU with this tuning");
-        bl->grid = per_cu * ctx->num_cus;
+    const size_t rest = lds_table_bytes(octaves, tune.max_res + 1) + dens_bytes_for(tune);
+    auto lds_for = [&](int32_t ring) { return lds_fixed_bytes(size_t(list_cap), size_t(ring), one_pass ? 0 : 1) + rest; };
+    auto fits_per_cu = [&](size_t lds, int* per_cu) -> int {
+        UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(bl->k),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+        *per_cu = 0;
+        UW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, reinterpret_cast<const void*>(bl->k),
+                                                                 tune.block, lds));
+        return UW_OK;
+    };
+    int per_cu = 0;
+    if (one_pass) {
+        bl->ring = list_cap;
+        bl->lds = lds_for(list_cap);
+        if (int rc = fits_per_cu(bl->lds, &per_cu)) return rc;
+    } else {  // the larger ring unless it costs resident workgroups that the launch's chunks would use
+        int per_max = 0;
+        if (int rc = fits_per_cu(lds_for(kRingMax), &per_max)) return rc;
+        if (int rc = fits_per_cu(lds_for(kRingMin), &per_cu)) return rc;
+        const bool no_loss = per_max >= per_cu || int64_t(nchunks) <= int64_t(per_max) * ctx->num_cus;
+        bl->ring = (per_max > 0 && no_loss) ? kRingMax : kRingMin;
+        if (bl->ring == kRingMax) per_cu = per_max;
+        bl->lds = lds_for(bl->ring);
+        if (int rc = fits_per_cu(bl->lds, &per_cu)) return rc;
     }
+    if (per_cu < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "batched kernel does not fit on a CU with this tuning");
+    bl->grid = tune.grid;
+    if (bl->grid == 0) bl->grid = per_cu * ctx->num_cus;
     // x-slab chunks keep their counting-pass densities in a per-workgroup scratch for the emission
     // pass (4 (n+1)^3 bytes of HBM traffic each way instead of a second density evaluation)
     bool slabs = false;
@@ -1031,7 +1053,7 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.max_res = tune.max_res;
     a.list_cap = list_cap_for(tune);
     const bool one_pass = one_pass_only(tune, a.list_cap);
-    a.ring_cap = one_pass ? a.list_cap : kRing;
+    a.ring_cap = bl.ring;
     a.has_lioff = one_pass ? 0 : 1;
     a.axis_stride = tune.max_res + 1;
     for (int n = 0; n <= kMaxN; ++n) a.sx[n] = n == 0 ? 1 : tune.slab[n];
