@@ -13,7 +13,7 @@ with max_res set to the workload's largest resolution (it sizes the launch's LDS
 every resolution's x-slab size to one LDS figure; the fastest shape is then refined one x-slab
 resolution of the workload at a time (coordinate descent over that resolution's planes per slab,
 within 4 of the current value), since the best slab of one size need not share the others'
-budget. The fastest shape per octave count goes to tuning.json, which `Perlin` applies when it
+budget; the descent repeats while a pass improves. The fastest shape per octave count goes to tuning.json, which `Perlin` applies when it
 creates a context. Every candidate's mesh is checked against the first candidate's (all shapes
 must produce identical meshes) before its time is accepted.
 
@@ -89,24 +89,33 @@ def _time_shape(perlin, coords, res, t, steps, stream, ref, log, label):
     return ms
 
 
-def refine_slabs(perlin, coords, res, best_ms, best, steps, stream, ref, log=print):
-    """Coordinate descent over the planes per slab of each x-slab resolution of the workload."""
+def refine_slabs(perlin, coords, res, best_ms, best, steps, stream, ref, log=print, passes=3):
+    """Coordinate descent over the planes per slab of each x-slab resolution of the workload,
+    repeated while a pass improves (one resolution's slab changes the LDS, and so the cell ring
+    and the occupancy, that the others' best values depend on)."""
     results = []
-    for n in sorted(set(int(r) for r in res)):
-        s0 = int(best["slab"][n])
-        if s0 >= n:  # one slab: nothing to refine
-            continue
-        for cand in range(max(1, s0 - 4), min(n - 1, s0 + 4) + 1):
-            if cand == s0:
+    tried = set()
+    for _ in range(passes):
+        improved = False
+        for n in sorted(set(int(r) for r in res)):
+            s0 = int(best["slab"][n])
+            if s0 >= n:  # one slab: nothing to refine
                 continue
-            t = dict(best, slab=list(best["slab"]))
-            t["slab"][n] = cand
-            ms = _time_shape(perlin, coords, res, t, steps, stream, ref, log, f"slab{n}={cand}")
-            if ms is None:
-                continue
-            results.append((ms, t))
-            if ms < best_ms:
-                best_ms, best = ms, t
+            for cand in range(max(1, s0 - 4), min(n - 1, s0 + 4) + 1):
+                t = dict(best, slab=list(best["slab"]))
+                t["slab"][n] = cand
+                key = tuple(t["slab"])
+                if cand == s0 or key in tried:
+                    continue
+                tried.add(key)
+                ms = _time_shape(perlin, coords, res, t, steps, stream, ref, log, f"slab{n}={cand}")
+                if ms is None:
+                    continue
+                results.append((ms, t))
+                if ms < best_ms:
+                    best_ms, best, improved = ms, t, True
+        if not improved:
+            break
     return best_ms, best, results
 
 
