@@ -139,7 +139,9 @@ static_assert(kOrderBuckets < 65536 && kPrepThreads <= 65536, "bucket and rank s
 struct OrderBufs {
     size_t keys_off, hist_off, bytes;  // byte offsets after the task records, total bytes
 };
-__host__ __device__ inline int32_t order_blocks(int32_t nchunks) { return (nchunks + kPrepThreads - 1) / kPrepThreads; }
+__host__ __device__ inline int32_t order_blocks(int32_t nchunks) {
+    return int32_t((int64_t(nchunks) + kPrepThreads - 1) / kPrepThreads);
+}
 inline OrderBufs order_bufs(int32_t nchunks) {
     OrderBufs o;
     o.keys_off = size_t(nchunks) * sizeof(uw_task);
