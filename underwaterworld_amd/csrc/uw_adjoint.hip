@@ -83,6 +83,8 @@ struct BackArgs {
     int64_t dens_stride;
     double* grad_amp;
     double* grad_off;
+    int32_t nsub;   // workgroups per chunk, each taking every nsub-th run of kBlock vertices
+    double* part;   // nsub > 1: per (chunk, share) parameter sums, [c][sub][OMAX + 3], added by k_backward_sum
 };
 
 template <int OMAX>
@@ -95,7 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     __shared__ double red[kBlock / 64][OMAX + 3];
 
     const int tid = threadIdx.x;
-    const int32_t c = blockIdx.x;
+    const int32_t c = int32_t(blockIdx.x / uint32_t(a.nsub)), sub = int32_t(blockIdx.x % uint32_t(a.nsub));
     const int32_t n = a.res[c];
     const int32_t O = a.f.octaves;
     const uw_chunk_info ci = a.info[c];
@@ -103,8 +105,12 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
                       ci.n_verts == 0;
     const bool want_params = a.grad_amp != nullptr || a.grad_off != nullptr;
     if (skip) {
-        if (tid < O && a.grad_amp) a.grad_amp[int64_t(c) * O + tid] = 0.0;
-        if (tid < 3 && a.grad_off) a.grad_off[int64_t(c) * 3 + tid] = 0.0;
+        if (a.nsub > 1) {
+            if (tid < OMAX + 3 && want_params) a.part[(int64_t(c) * a.nsub + sub) * (OMAX + 3) + tid] = 0.0;
+        } else {
+            if (tid < O && a.grad_amp) a.grad_amp[int64_t(c) * O + tid] = 0.0;
+            if (tid < 3 && a.grad_off) a.grad_off[int64_t(c) * 3 + tid] = 0.0;
+        }
         return;
     }
     const int32_t L = n + 1;
@@ -141,7 +147,7 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     for (int k = 0; k < OMAX + 3; ++k) acc[k] = 0.0;
     float* gd = a.grad_dens ? a.grad_dens + int64_t(c) * a.dens_stride : nullptr;
     const int32_t LL = L * L;
-    for (uint32_t i = tid; i < ci.n_verts; i += kBlock) {
+    for (uint32_t i = uint32_t(sub) * kBlock + tid; i < ci.n_verts; i += uint32_t(a.nsub) * kBlock) {
         const uint64_t v = ci.vert_off + i;
         const uw_vertex_tape tv = a.tape[v];
         const int32_t diff = int32_t(tv.b) - int32_t(tv.a);
@@ -199,12 +205,63 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     if (tid < OMAX + 3) {
         double t = 0.0;
         for (int w = 0; w < kBlock / 64; ++w) t += red[w][tid];
-        if (tid < OMAX) {
+        if (a.nsub > 1) {
+            a.part[(int64_t(c) * a.nsub + sub) * (OMAX + 3) + tid] = t;
+        } else if (tid < OMAX) {
             if (tid < O && a.grad_amp) a.grad_amp[int64_t(c) * O + tid] = t;
         } else if (a.grad_off) {
             a.grad_off[int64_t(c) * 3 + (tid - OMAX)] = t;
         }
     }
+}
+
+// Parameter sums of the nsub workgroups of each chunk, in workgroup order: one thread per (chunk,
+// parameter).
+template <int OMAX>
+__global__ void __launch_bounds__(256) k_backward_sum(const double* __restrict__ part, int32_t nchunks, int32_t nsub,
+                                                     int32_t O, double* grad_amp, double* grad_off) {
+    const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= int64_t(nchunks) * (OMAX + 3)) return;
+    const int64_t c = e / (OMAX + 3);
+    const int k = int(e - c * (OMAX + 3));
+    double t = 0.0;
+    for (int32_t s = 0; s < nsub; ++s) t += part[(c * nsub + s) * (OMAX + 3) + k];
+    if (k < OMAX) {
+        if (k < O && grad_amp) grad_amp[c * O + k] = t;
+    } else if (grad_off) {
+        grad_off[c * 3 + (k - OMAX)] = t;
+    }
+}
+
+template <int OMAX>
+int launch_backward(uw_ctx* ctx, BackArgs a, int32_t nchunks, hipStream_t st) {
+    // several workgroups per chunk when the launch has fewer chunks than ~4 per CU: each takes a
+    // share of the chunk's vertices, and the shares' parameter sums are added afterwards
+    const bool want_params = a.grad_amp != nullptr || a.grad_off != nullptr;
+    int32_t nsub = 1;
+    while (nsub < 4 && int64_t(nchunks) * nsub * 2 <= int64_t(ctx->num_cus) * 4) nsub *= 2;
+    a.nsub = nsub;
+    a.part = nullptr;
+    if (nsub > 1 && want_params) {
+        const size_t bytes = size_t(nchunks) * size_t(nsub) * (OMAX + 3) * sizeof(double);
+        if (bytes > ctx->adj_part_bytes) {
+            (void)hipFree(ctx->d_adj_part);
+            ctx->d_adj_part = nullptr;
+            ctx->adj_part_bytes = 0;
+            UW_HIP(ctx, hipMalloc(&ctx->d_adj_part, bytes));
+            ctx->adj_part_bytes = bytes;
+        }
+        a.part = ctx->d_adj_part;
+    }
+    const uint64_t blocks = uint64_t(nchunks) * uint64_t(nsub);
+    if (blocks > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
+    hipLaunchKernelGGL(k_backward<OMAX>, dim3(uint32_t(blocks)), dim3(kBlock), 0, st, a);
+    if (nsub > 1 && want_params) {
+        const int64_t work = int64_t(nchunks) * (OMAX + 3);
+        hipLaunchKernelGGL(k_backward_sum<OMAX>, dim3(uint32_t((work + 255) / 256)), dim3(256), 0, st, a.part, nchunks,
+                           nsub, a.f.octaves, a.grad_amp, a.grad_off);
+    }
+    return UW_OK;
 }
 
 }  // namespace
@@ -259,9 +316,11 @@ int uw_adjoint_backward(uw_ctx* ctx, const int32_t* coords, const int32_t* res, 
     hipStream_t st = static_cast<hipStream_t>(stream);
     const int O = a.f.octaves;
     if (int rc = uw_prof_start(ctx, st)) return rc;
-    if (O <= 4) hipLaunchKernelGGL(k_backward<4>, dim3(uint32_t(nchunks)), dim3(kBlock), 0, st, a);
-    else if (O <= 8) hipLaunchKernelGGL(k_backward<8>, dim3(uint32_t(nchunks)), dim3(kBlock), 0, st, a);
-    else hipLaunchKernelGGL(k_backward<16>, dim3(uint32_t(nchunks)), dim3(kBlock), 0, st, a);
+    int lrc = UW_OK;
+    if (O <= 4) lrc = launch_backward<4>(ctx, a, nchunks, st);
+    else if (O <= 8) lrc = launch_backward<8>(ctx, a, nchunks, st);
+    else lrc = launch_backward<16>(ctx, a, nchunks, st);
+    if (lrc) return lrc;
     if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;

@@ -232,6 +232,7 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(ctx->d_batch_order);
     (void)hipFree(ctx->d_batch_ztab);
     (void)hipFree(ctx->d_batch_dscr);
+    (void)hipFree(ctx->d_adj_part);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
     delete ctx;
 }

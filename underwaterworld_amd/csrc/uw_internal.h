@@ -63,6 +63,8 @@ struct uw_ctx {
     size_t batch_ztab_bytes = 0;
     float* d_batch_dscr = nullptr;  // per-workgroup density scratch of x-slab chunks (eager batched path)
     size_t batch_dscr_bytes = 0;
+    double* d_adj_part = nullptr;  // adjoint backward: per-(chunk, vertex share) parameter sums
+    size_t adj_part_bytes = 0;
     uint8_t perm[256];
     uw_tables host_tables;
     uw_tables* d_tables = nullptr;
