@@ -218,7 +218,7 @@ def main():
     volume = None
     if not args.no_volume and not args.no_adjoint:
         from underwaterworld_amd.adjoint import ChunkMesher
-        pv = params.preset("volume512")
+        pv = params.preset("volume_bricks")
         bricks = params.volume_bricks(16)
         pvol = Perlin(pv)
         tv = get_batch_tuning(pvol)

@@ -232,7 +232,11 @@ int uw_set_batch_tuning(uw_ctx* ctx, const uw_batch_tuning* t);
  * build of a whole chunk set. coords: device int32[3 n], res: device int32[n] cells per axis
  * (1..32, per chunk). Output as for uw_build_chunks_plain; chunk ranges are placed in completion
  * order (each chunk's record gives its offsets; contents are deterministic). Asynchronous on
- * `stream`; no host synchronisation, so the call can be captured in a graph. */
+ * `stream`, with no host synchronisation. Its launches bind scratch owned by the context, which a
+ * later call with more chunks reallocates, so this entry point must not be captured into a graph:
+ * on a capturing stream it returns UW_ERR_INVALID_ARG (as do uw_build_chunks_plain,
+ * uw_sample_density, uw_build_chunks_batched_tape and uw_adjoint_backward). Graph use goes through
+ * uw_batched_graph_create, whose graph owns its buffers. */
 int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
                             const uw_mesh_out* out, void* stream);
 
@@ -270,6 +274,39 @@ int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32
 int uw_adjoint_backward(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
                         const uw_chunk_info* info, const uw_vertex_tape* tape, const float* grad_verts,
                         float* grad_dens, int64_t dens_stride, double* grad_amp, double* grad_off, void* stream);
+
+/* ---- Single large volume: one chunk meshed as one grid --------------------------------------- */
+
+#define UW_VOLUME_MAX_RESOLUTION 640 /* cells per axis; keeps every vertex id and triangle count < 2^32 */
+
+/* Chunk::new(pos) + build_full (chunk.rs:89-103,266-313) for ONE chunk of n cells per axis
+ * (1 <= n <= UW_VOLUME_MAX_RESOLUTION) at chunk coordinate (cx, cy, cz), with the context's chunk
+ * extent: samples at ((f64) i * (f64) step + extent * c) for i in [0, n], step = (f32) extent / (f32) n
+ * (chunk.rs:105-129), cube cases, lerps and the first-seen vertex numbering over the whole grid
+ * (chunk.rs:141-243): the result the reference's builder gives for that chunk, as one mesh in one
+ * index space. The per-chunk paths stop at 32 cells per axis; this one is tiled internally (32^3
+ * tiles with a one-cell halo for the numbering across tile faces).
+ * out: as for uw_build_chunks_plain with one chunk (info[0] receives the record, vert_off =
+ * ind_off = 0). tape (optional, device, out->vert_cap entries): per vertex the endpoints' sample
+ * indices x (n+1)^2 + y (n+1) + z and their densities, for uw_volume_backward. The density field,
+ * the tables and the row counts live in a context-owned scratch of uw_volume_scratch_bytes.
+ * Asynchronous on `stream`; refuses a capturing stream (UW_ERR_INVALID_ARG). */
+int uw_build_volume(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, const uw_mesh_out* out,
+                    uw_vertex_tape* tape, void* stream);
+
+/* Backward of uw_build_volume (no counterpart in the reference): as uw_adjoint_backward for the
+ * volume's one chunk, with the octave noise recomputed at every vertex's endpoints (no stored
+ * field). info: the forward's device record; tape / grad_verts: device, laid out like its vertices;
+ * grad_dens: device float[(n+1)^3] (index as the tape's) or NULL, accumulated; grad_amp: device
+ * double[octaves] or NULL; grad_off: device double[3] or NULL (both written, summed in a fixed
+ * order). */
+int uw_volume_backward(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, const uw_chunk_info* info,
+                       const uw_vertex_tape* tape, const float* grad_verts, float* grad_dens, double* grad_amp,
+                       double* grad_off, void* stream);
+
+/* Bytes of the context-owned scratch that uw_build_volume / uw_volume_backward use for n cells
+ * per axis (the density field is 4 (n+1)^3 of it); 0 for invalid arguments. */
+uint64_t uw_volume_scratch_bytes(const uw_ctx* ctx, int32_t n);
 
 /* The batched build captured as an instantiated HIP graph over fixed buffers; relaunch it after
  * changing the contents (not the addresses or count) of coords/res. */

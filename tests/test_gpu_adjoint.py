@@ -124,3 +124,113 @@ def test_adjoint_matches_finite_differences():
         worst = max(worst, rel)
     print(f"\nadjoint vs central differences (h=1e-3, Richardson): max relative error {worst:.3g}")
     assert worst <= 1e-4
+
+
+def _grad_z(mesher, rec):
+    import torch
+    gv = torch.zeros((mesher.vert_cap, 3), dtype=torch.float32, device="cuda")
+    nv = int(rec.batch.totals()["n_verts"])
+    gv[:nv, 2] = 1.0
+    gv[:nv, 0] = 0.5
+    return gv
+
+
+def test_two_outstanding_forwards_on_one_mesher():
+    # two forwards with different fields on one mesher, backward of the first after the second:
+    # each backward must use its own forward's tape and field (compared with separate meshers),
+    # and neither call may change the context's own field
+    import torch
+    from underwaterworld_amd import Perlin, build_chunks_batched
+    from underwaterworld_amd.adjoint import ChunkMesher
+    p = PR.preset("adjoint24")
+    coords, res = COORDS[:4], [24, 12, 12, 8]
+    a1 = np.array(p["amplitudes"], np.float64)
+    a2 = a1 * np.array([1.0, 0.7, 1.3, 0.9, 1.1, 0.8])
+    o2 = np.array([0.3, -0.2, 0.1])
+    shared = ChunkMesher(Perlin(p), coords, res, want_dens_grad=True)
+    before = build_chunks_batched(shared.perlin, coords, res)
+    r1 = shared.forward(amplitudes=a1)
+    r2 = shared.forward(amplitudes=a2, offset=o2)
+    g1 = shared.backward(_grad_z(shared, r1), record=r1)
+    g2 = shared.backward(_grad_z(shared, r2), record=r2)
+    after = build_chunks_batched(shared.perlin, coords, res)
+    torch.cuda.synchronize()
+    assert np.allclose(shared.perlin.field()[0], a1) and np.all(shared.perlin.field()[1] == 0)
+    for want, got in ((before, after),):
+        assert np.array_equal(want.totals(), got.totals())
+    for amps, off, rec, g in ((a1, None, r1, g1), (a2, o2, r2, g2)):
+        sep = ChunkMesher(Perlin(p), coords, res, want_dens_grad=True)
+        sep.set_field(amps, off)
+        rs = sep.forward()
+        gs = sep.backward(_grad_z(sep, rs), record=rs)
+        torch.cuda.synchronize()
+        ti, si = rec.batch.info(), rs.batch.info()
+        assert np.array_equal(ti["n_verts"], si["n_verts"]) and np.array_equal(ti["n_inds"], si["n_inds"])
+        for i in range(len(coords)):
+            assert np.array_equal(rec.batch.chunk_mesh(i, ti)[0], rs.batch.chunk_mesh(i, si)[0])
+        assert np.allclose(g.grad_amp.cpu().numpy(), gs.grad_amp.cpu().numpy(), rtol=1e-12, atol=0)
+        assert np.allclose(g.grad_off.cpu().numpy(), gs.grad_off.cpu().numpy(), rtol=1e-12, atol=0)
+        assert np.allclose(g.grad_dens.cpu().numpy(), gs.grad_dens.cpu().numpy(), rtol=1e-5, atol=1e-7)
+    assert not np.allclose(g1.grad_amp.cpu().numpy(), g2.grad_amp.cpu().numpy())
+
+
+@pytest.mark.parametrize("copies", [87, 65])  # 6 chunks x 87 > 2 x 256 CUs: one workgroup per chunk;
+def test_backward_launch_forms(copies):     # 6 x 65 in (256, 512]: two workgroups per chunk
+    # the backward splits a chunk's vertices over 4, 2 or 1 workgroups by the launch's chunk count;
+    # every form gives each chunk the gradients of the 4-way split run on the 6 chunks alone
+    import torch
+    from underwaterworld_amd import Perlin
+    from underwaterworld_amd.adjoint import ChunkMesher
+    p = PR.preset("adjoint24")
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    nsub = lambda n: 4 if n <= cus else (2 if n <= 2 * cus else 1)  # noqa: E731  (uw_adjoint.hip)
+    big = COORDS * copies
+    assert nsub(len(COORDS)) == 4 and nsub(len(big)) == (1 if copies == 87 else 2), (cus, len(big))
+    runs = []
+    for cs in (COORDS, big):
+        m = ChunkMesher(Perlin(p), cs, 8, want_dens_grad=True)
+        rec = m.forward()
+        g = m.backward(_grad_z(m, rec), record=rec)
+        torch.cuda.synchronize()
+        runs.append((g.grad_amp.cpu().numpy(), g.grad_off.cpu().numpy(), g.grad_dens.cpu().numpy()))
+    (a0, o0, d0), (a1, o1, d1) = runs
+    assert np.any(a0 != 0)
+    for k in range(len(big)):
+        j = k % len(COORDS)
+        assert np.allclose(a1[k], a0[j], rtol=1e-12, atol=1e-300), k
+        assert np.allclose(o1[k], o0[j], rtol=1e-12, atol=1e-300), k
+        assert np.allclose(d1[k], d0[j], rtol=1e-5, atol=1e-6), k
+
+
+def test_capturing_stream_is_refused():
+    # entry points whose launches bind context-owned scratch refuse a capturing stream (a later call
+    # may reallocate that scratch under a captured graph); nothing is recorded into the graph
+    import ctypes
+    import torch
+    from underwaterworld_amd import Perlin, _lib
+    from underwaterworld_amd.engine import _alloc_out, _coords_tensor, _mesh_out, _res_tensor
+    perlin = Perlin(PR.preset("chunks12"))
+    ct = _coords_tensor(COORDS, torch.device("cuda", 0))
+    rt = _res_tensor(12, len(COORDS), torch.device("cuda", 0))
+    verts, inds, info, totals = _alloc_out(len(COORDS), 60000, 200000, torch.device("cuda", 0))
+    out = _mesh_out(verts, inds, info, totals, 60000, 200000)
+    L = _lib.lib()
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    rcs = {}
+    with torch.cuda.graph(g, stream=s):
+        sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        rcs["batched"] = L.uw_build_chunks_batched(perlin.ctx, ctypes.c_void_p(ct.data_ptr()),
+                                                   ctypes.c_void_p(rt.data_ptr()), len(COORDS), ctypes.byref(out), sp)
+        rcs["plain"] = L.uw_build_chunks_plain(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), len(COORDS), 12,
+                                               ctypes.byref(out), ctypes.c_void_p(0), sp)
+        verts.zero_()  # the capture holds one ordinary node
+    torch.cuda.synchronize()
+    assert rcs == {"batched": _lib.UW_ERR_INVALID_ARG, "plain": _lib.UW_ERR_INVALID_ARG}, rcs
+    assert b"capturing" in L.uw_last_error(perlin.ctx)
+    # outside capture the same calls work
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    assert L.uw_build_chunks_batched(perlin.ctx, ctypes.c_void_p(ct.data_ptr()), ctypes.c_void_p(rt.data_ptr()),
+                                     len(COORDS), ctypes.byref(out), sp) == _lib.UW_OK
+    torch.cuda.synchronize()

@@ -1,0 +1,252 @@
+"""GPU checks of the workloads at their full sizes, not subsets:
+
+  * the 8192-chunk mixed-resolution launch (8^3/12^3/24^3/32^3, 6 octaves, SplitMix64 seed 7)
+    equals the plain path run per resolution, chunk by chunk, with consistent totals;
+  * the 256-chunk 24^3 adjoint workload: its forward (batched build with a vertex tape) equals the
+    plain path chunk by chunk, and its per-chunk amplitude and offset gradients agree with the
+    oracle's FP64 analytic gradient (same densities and cube cases) on a sample of chunks, to the
+    1e-4 relative bar;
+  * the 8-octave kernels (the 512^3 volume's field): densities of surface, blank and solid 32^3
+    bricks against the oracle, the batched build equal to the plain path and to the oracle's
+    mesher brick by brick, and the 8-octave adjoint against FP64 and central differences.
+
+Whole-buffer copies to the host keep the per-chunk comparisons fast at 8192 chunks.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from parity import DENS_TOL
+from underwaterworld_amd import params as PR
+
+pytestmark = pytest.mark.gpu
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+class HostBatch:
+    """One host copy of a MeshBatch's buffers, sliced per chunk."""
+
+    def __init__(self, batch):
+        self.info = batch.info()
+        self.verts = batch.verts.cpu().numpy()
+        ix = batch.inds.cpu().numpy()
+        self.inds = ix.view(np.uint16 if ix.dtype == np.int16 else np.uint32)
+        self.totals = batch.totals()
+
+    def mesh(self, i):
+        r = self.info[i]
+        v0, nv, i0, ni = int(r["vert_off"]), int(r["n_verts"]), int(r["ind_off"]), int(r["n_inds"])
+        return self.verts[v0:v0 + nv], self.inds[i0:i0 + ni]
+
+
+def _plain_host(perlin, coords, res):
+    """{chunk index: (info record, verts, inds)} of the plain path, one launch per resolution."""
+    from underwaterworld_amd import build_chunks_plain
+    out = {}
+    res = np.asarray(res)
+    for n in sorted(set(res.tolist())):
+        idx = np.nonzero(res == n)[0]
+        b = build_chunks_plain(perlin, [coords[i] for i in idx], int(n))
+        _sync()
+        h = HostBatch(b)
+        for j, i in enumerate(idx):
+            v, ix = h.mesh(j)
+            out[int(i)] = (h.info[j], v, ix)
+    return out
+
+
+def _assert_equal_chunkwise(h, ref, nchunks):
+    for i in range(nchunks):
+        ri, rv, rx = ref[i]
+        bi = h.info[i]
+        assert (bi["flags"], bi["n_verts"], bi["n_inds"], bi["n_active"]) == \
+               (ri["flags"], ri["n_verts"], ri["n_inds"], ri["n_active"]), i
+        v, ix = h.mesh(i)
+        assert np.array_equal(v, rv) and np.array_equal(ix, rx), i
+    t = h.totals
+    assert t["overflow"] == 0
+    assert int(t["n_verts"]) == sum(int(ref[i][0]["n_verts"]) for i in ref)
+    assert int(t["n_inds"]) == sum(int(ref[i][0]["n_inds"]) for i in ref)
+    assert int(t["n_nonblank"]) == sum(int(ref[i][0]["n_inds"]) > 0 for i in ref)
+
+
+def test_mixed_8192_full_launch_equals_plain():
+    from underwaterworld_amd import Perlin, build_chunks_batched
+    p = PR.preset("mixed")
+    perlin = Perlin(p)
+    coords, res = PR.mixed_workload(8192, 7)
+    ref = _plain_host(perlin, coords, res)
+    b = build_chunks_batched(perlin, coords, res)
+    _sync()
+    h = HostBatch(b)
+    _assert_equal_chunkwise(h, ref, len(coords))
+    # every chunk's ranges lie inside the totals and do not overlap
+    info = h.info
+    nz = info["n_inds"] > 0
+    v_end = (info["vert_off"] + info["n_verts"])[nz]
+    assert v_end.max() == h.totals["n_verts"]
+    order = np.argsort(info["vert_off"][nz])
+    assert np.all(info["vert_off"][nz][order][1:] >= v_end[order][:-1])
+    print(f"\nmixed 8192: nonblank {int(h.totals['n_nonblank'])}, verts {int(h.totals['n_verts'])}, "
+          f"inds {int(h.totals['n_inds'])}")
+
+
+CORNERS = [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)]
+
+
+def _cases(d, n, iso):
+    D = d.reshape(n + 1, n + 1, n + 1) < np.float32(iso)
+    c = np.zeros((n, n, n), np.uint8)
+    for i, (a, b, e) in enumerate(CORNERS):
+        c |= (D[a:a + n, b:b + n, e:e + n].astype(np.uint8) << i)
+    return c.ravel()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def _plane_grad(mesher, rec, target):
+    """dL/dverts of the workload's loss, sum over vertices of (v_z - chunk_origin_z - target)^2."""
+    import torch
+    from underwaterworld_amd.adjoint import plane_loss
+    nv = int(rec.batch.totals()["n_verts"])
+    v = rec.batch.verts[:nv].clone().requires_grad_(True)
+    loss = plane_loss(v, mesher, target, rec)
+    (g,) = torch.autograd.grad(loss, v)
+    return g
+
+
+def test_adjoint_256_full_workload():
+    from underwaterworld_amd import Perlin, sample_density
+    from underwaterworld_amd.adjoint import ChunkMesher
+    p = PR.preset("adjoint24")
+    coords = [(x, y, z) for x in range(-4, 4) for y in range(-4, 4) for z in range(-3, 1)]
+    perlin = Perlin(p)
+    m = ChunkMesher(perlin, coords, 24, want_dens_grad=False)
+    rec = m.forward()
+    _sync()
+    h = HostBatch(rec.batch)
+    ref = _plain_host(perlin, coords, [24] * len(coords))
+    _assert_equal_chunkwise(h, ref, len(coords))
+    target = 0.25 * p["chunk_extent"]
+    g = m.backward(_plane_grad(m, rec, target), record=rec)
+    _sync()
+    ga, go = g.grad_amp.cpu().numpy(), g.grad_off.cpu().numpy()
+    assert np.all(np.isfinite(ga)) and np.all(np.isfinite(go))
+    blank = h.info["n_verts"] == 0
+    assert np.all(ga[blank] == 0) and np.all(go[blank] == 0)
+    # FP64 analytic gradient on a sample of surface chunks (same densities and cube cases)
+    surface = np.nonzero(~blank)[0]
+    pick = surface[np.linspace(0, len(surface) - 1, 6).astype(int)]
+    dens, _ = sample_density(perlin, [coords[i] for i in pick], 24)
+    _sync()
+    dens = dens.cpu().numpy()
+    worst_a = worst_o = 0.0
+    for j, i in enumerate(pick):
+        r = O.adjoint_chunk(p, coords[i], 24, target, cases=_cases(dens[j], 24, p["iso"]),
+                            rho=dens[j].astype(np.float64))
+        assert r["nverts"] == h.info[i]["n_verts"]
+        worst_a = max(worst_a, _rel(ga[i], r["g_amp"]))
+        worst_o = max(worst_o, _rel(go[i], r["g_off"]))
+    print(f"\nadjoint 256 x 24^3: {int((~blank).sum())} surface chunks, {int(h.totals['n_verts'])} vertices; "
+          f"sampled chunks vs FP64: amp {worst_a:.3g}, off {worst_o:.3g}")
+    assert worst_a <= 1e-4 and worst_o <= 1e-4
+
+
+# --- the 8-octave field of the 512^3 volume, on 32^3 bricks -----------------------------------
+
+def _volume_bricks_sample():
+    """Bricks of the volume preset: surface bricks near the bias zero, deep (solid) and high
+    (blank) ones."""
+    return [(0, 0, 0), (3, 5, 0), (7, 2, 1), (15, 15, 0), (8, 8, 2), (1, 14, 3),  # near the surface
+            (4, 4, 8), (9, 1, 12), (15, 0, 15), (2, 2, 6),                         # far above
+            (5, 5, -6), (0, 12, -10)]                                               # far below
+
+
+def test_volume_field_bricks_8_octaves():
+    from underwaterworld_amd import Perlin, build_chunks_batched, get_batch_tuning, sample_density, set_batch_tuning
+    p = PR.preset("volume_bricks")
+    assert p["octaves"] == 8
+    perlin = Perlin(p)
+    t = get_batch_tuning(perlin)
+    t["max_res"] = 32
+    set_batch_tuning(perlin, t)
+    bricks = _volume_bricks_sample()
+    dens, _ = sample_density(perlin, bricks, 32)
+    _sync()
+    d = dens.cpu().numpy()
+    worst, kinds = 0.0, {"surface": 0, "blank": 0, "solid": 0}
+    for k, c in enumerate(bricks):
+        dref = O.density_ref(p, c, 32)
+        worst = max(worst, float(np.max(np.abs(d[k].astype(np.float64) - dref))))
+        kinds["blank" if dref.min() > p["iso"] else "solid" if dref.max() < p["iso"] else "surface"] += 1
+    assert all(v > 0 for v in kinds.values()), kinds
+    ref = _plain_host(perlin, bricks, [32] * len(bricks))
+    b = build_chunks_batched(perlin, bricks, 32)
+    _sync()
+    h = HostBatch(b)
+    _assert_equal_chunkwise(h, ref, len(bricks))
+    for k, c in enumerate(bricks):
+        if h.info[k]["flags"] & 2:
+            continue
+        mg = O.mesh_ref(p, c, 32, d[k])
+        v, ix = h.mesh(k)
+        assert np.array_equal(ix, mg["inds"]) and np.array_equal(v, mg["verts"]), c
+    print(f"\n8-octave bricks {kinds}: max|drho| = {worst:.3g}")
+    assert worst <= DENS_TOL
+
+
+def test_volume_field_adjoint_8_octaves():
+    from underwaterworld_amd import Perlin, sample_density
+    from underwaterworld_amd.adjoint import ChunkMesher
+    p = PR.preset("volume_bricks")
+    perlin = Perlin(p)
+    bricks = [(0, 0, 0), (3, 5, 0)]
+    m = ChunkMesher(perlin, bricks, 32, want_dens_grad=True)
+    rec = m.forward()
+    _sync()
+    target = 0.25 * p["chunk_extent"]
+    g = m.backward(_plane_grad(m, rec, target), record=rec)
+    _sync()
+    ga, go, gd = g.grad_amp.cpu().numpy(), g.grad_off.cpu().numpy(), g.grad_dens.cpu().numpy()
+    dens, _ = sample_density(perlin, bricks, 32)
+    _sync()
+    dens = dens.cpu().numpy()
+    info = rec.batch.info()
+    h = 1e-3
+    worst = {"amp": 0.0, "off": 0.0, "dens": 0.0, "fd": 0.0}
+    for i, c in enumerate(bricks):
+        assert info[i]["n_verts"] > 0
+        cs = _cases(dens[i], 32, p["iso"])
+        r = O.adjoint_chunk(p, c, 32, target, cases=cs, rho=dens[i].astype(np.float64))
+        worst["amp"] = max(worst["amp"], _rel(ga[i], r["g_amp"]))
+        worst["off"] = max(worst["off"], _rel(go[i], r["g_off"]))
+        worst["dens"] = max(worst["dens"], _rel(gd[i][: 33 ** 3], r["g_dens"]))
+
+        def f_at(amp=None, ax=None):
+            def f(dl):
+                q = dict(p, amplitudes=list(p["amplitudes"]))
+                if amp is not None:
+                    q["amplitudes"][amp] += dl
+                if ax is not None:
+                    o = [0.0, 0.0, 0.0]
+                    o[ax] = dl
+                    q["offset"] = tuple(o)
+                return O.adjoint_chunk(q, c, 32, target, cases=cs, grads=False)["loss"]
+            return f
+
+        def rich(f):
+            c1 = (f(h) - f(-h)) / (2 * h)
+            c2 = (f(h / 2) - f(-h / 2)) / h
+            return (4 * c2 - c1) / 3
+
+        fd = np.array([rich(f_at(amp=j)) for j in (0, 3, 7)] + [rich(f_at(ax=k)) for k in range(3)])
+        gk = np.concatenate([ga[i][[0, 3, 7]], go[i]])
+        worst["fd"] = max(worst["fd"], float(np.max(np.abs(gk - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max()))))
+    print(f"\n8-octave adjoint on 32^3 bricks: {worst}")
+    assert worst["amp"] <= 1e-4 and worst["off"] <= 1e-4 and worst["dens"] <= 1e-4 and worst["fd"] <= 1e-4

@@ -1,0 +1,293 @@
+"""GPU checks of the single-volume path (uw_build_volume / uw_volume_backward): one chunk of n cells
+per axis meshed as one grid with one index space (chunk.rs:105-264 run on that chunk).
+
+  * equal, bit for bit, to the per-chunk plain path on chunks it can build (n <= 32), for every
+    preset and wire format (VertColor, u16, collision records);
+  * at n = 40 / 64 (partial and whole 32^3 tiles) equal to the oracle's mesher on the engine's own
+    densities, and to the oracle's reference-faithful densities within 1e-6 (case bits differ only
+    at ties), positions within the parity bar;
+  * the full 512^3 workload (513^3 samples, seed 1234, 8 octaves): size-independent properties
+    against restatements of the rules -- every tape density within 1e-6 of the oracle's iso_at at
+    that sample, every record once (ordered corner pairs unique), vertices numbered in the scan
+    order of their owner cells, triangles in cell order, each triangle's records on edges of its
+    cell, indices in range, and every vertex used;
+  * the adjoint against the oracle's FP64 gradient and central differences, and equal to the
+    batched adjoint on a chunk both can build.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from parity import DENS_TOL, TIE_TOL, edge_density_gaps, position_check, tie_corners
+from test_tables import packed_rows
+from underwaterworld_amd import params as PR
+
+pytestmark = pytest.mark.gpu
+
+CORNERS = [(0, 0, 0), (1, 0, 0), (1, 0, 1), (0, 0, 1), (0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)]
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _host(batch):
+    info = batch.info()[0]
+    nv, ni = int(info["n_verts"]), int(info["n_inds"])
+    v = batch.verts[:nv].cpu().numpy()
+    ix = batch.inds[:ni].cpu().numpy()
+    return info, v, ix.view(np.uint16 if ix.dtype == np.int16 else np.uint32)
+
+
+CHUNK_CASES = [("reference", 12, (0, 0, -1)), ("reference", 12, (3, 1, -1)), ("chunks12", 12, (0, 0, -1)),
+               ("mixed", 24, (3, 1, -1)), ("mixed", 32, (-9, 4, 0)), ("adjoint24", 7, (1, 2, 0)),
+               ("volume_bricks", 32, (3, 5, 0)), ("reference", 1, (0, 0, -1))]
+
+
+@pytest.mark.parametrize("preset,n,coord", CHUNK_CASES, ids=[f"{c[0]}-n{c[1]}" for c in CHUNK_CASES])
+def test_volume_equals_plain_chunk(preset, n, coord):
+    from underwaterworld_amd import Perlin, build_chunks_plain, build_volume
+    perlin = Perlin(PR.preset(preset))
+    ref = build_chunks_plain(perlin, [coord], n)
+    vol = build_volume(perlin, n, coord)
+    _sync()
+    ri, rv, rx = ref.info()[0], *ref.chunk_mesh(0)
+    vi, vv, vx = _host(vol)
+    assert (vi["flags"], vi["n_verts"], vi["n_inds"], vi["n_active"]) == \
+           (ri["flags"], ri["n_verts"], ri["n_inds"], ri["n_active"])
+    assert np.array_equal(vv, rv) and np.array_equal(vx, rx)
+
+
+def test_volume_wire_formats_equal_plain():
+    from underwaterworld_amd import Perlin, build_chunks_plain, build_volume
+    perlin = Perlin(PR.preset("reference"))
+    for coord in [(0, 0, -1), (-4, 3, -1)]:
+        ref = build_chunks_plain(perlin, [coord], 12, colors=True, u16=True, collision=True)
+        vol = build_volume(perlin, 12, coord, colors=True, u16=True, collision=True)
+        _sync()
+        ri = ref.info()[0]
+        vi, vv, vx = _host(vol)
+        rv, rx = ref.chunk_mesh(0)
+        assert vi["n_inds"] == ri["n_inds"] and vi["n_inds"] > 0
+        assert np.array_equal(vv, rv) and np.array_equal(vx, rx)
+        rc, rn = ref.chunk_collision(0)
+        nt = int(vi["n_inds"]) // 3
+        assert np.array_equal(vol.tri_cells[:nt].cpu().numpy().view(np.uint32), rc)
+        assert np.array_equal(vol.tri_normals[:nt].cpu().numpy(), rn)
+
+
+@pytest.mark.parametrize("n,coord", [(40, (0, 0, 0)), (64, (0, 0, 0)), (45, (1, -1, 0))])
+def test_volume_against_oracle(n, coord):
+    from underwaterworld_amd import Perlin, build_volume, sample_density
+    p = PR.preset("volume512", chunk_extent=n, resolution=n)
+    perlin = Perlin(p)
+    vol = build_volume(perlin, n, coord)
+    dens, _ = sample_density(perlin, [coord], n)  # the same evaluator over the same grid
+    _sync()
+    d_gpu = dens.cpu().numpy()[0]
+    info, v, ix = _host(vol)
+    # the engine's mesher over its own densities: bit-identical to the oracle's
+    mg = O.mesh_ref(p, coord, n, d_gpu)
+    assert info["n_verts"] == len(mg["verts"]) and info["n_active"] == mg["n_active"]
+    assert np.array_equal(ix, mg["inds"]) and np.array_equal(v, mg["verts"])
+    # against the reference-faithful oracle
+    dref = O.density_ref(p, coord, n)
+    err = float(np.max(np.abs(d_gpu.astype(np.float64) - dref)))
+    mr = O.mesh_ref(p, coord, n, dref)
+    bits, ok = tie_corners(mg["cases"], mr["cases"], dref, n, float(np.float32(p["iso"])))
+    assert ok
+    msg = f"\nvolume n={n} {coord}: {len(v)} vertices, {len(ix)} indices, max|drho|={err:.3g}, tie bits {bits}"
+    if bits == 0:
+        assert np.array_equal(ix, mr["inds"])
+        h = float(np.float32(p["chunk_extent"]) / np.float32(n))
+        gaps = edge_density_gaps(mr["verts"], mr["inds"], mr["cases"], dref, n, packed_rows())
+        e, nill, okp = position_check(v, mr["verts"], gaps, h)
+        assert okp
+        msg += f", max|dp|/h={e:.3g}, ill-conditioned {nill}"
+    print(msg)
+    assert err <= DENS_TOL
+
+
+def test_volume_overflow_reports_sizes():
+    from underwaterworld_amd import Perlin, _lib, build_volume
+    p = PR.preset("volume512", chunk_extent=40, resolution=40)
+    perlin = Perlin(p)
+    full = build_volume(perlin, 40)
+    small = build_volume(perlin, 40, vert_cap=100, ind_cap=100)
+    _sync()
+    t, ts = full.totals(), small.totals()
+    assert ts["overflow"] == 1 and small.info()[0]["flags"] & _lib.CHUNK_OVERFLOW
+    assert ts["n_verts"] == t["n_verts"] and ts["n_inds"] == t["n_inds"] and t["overflow"] == 0
+
+
+def _owner_cell_of_record(a, b, L, n):
+    """First cell in scan order (x, y, z) that contains the edge between samples a and b (linear
+    x L^2 + y L + z): the sharer with the smallest coordinates along the two other axes."""
+    a = a.astype(np.int64)
+    b = b.astype(np.int64)
+    lo = np.minimum(a, b)
+    x, y, z = lo // (L * L), (lo // L) % L, lo % L
+    d = np.abs(b - a)
+    # the edge runs along x (d = L^2), y (d = L) or z (d = 1); sharers differ in the two other axes
+    cx = np.where(d == L * L, x, np.maximum(x - 1, 0))
+    cy = np.where(d == L, y, np.maximum(y - 1, 0))
+    cz = np.where(d == 1, z, np.maximum(z - 1, 0))
+    # a sharer must contain the edge: the lower neighbour is used only when the edge is not on the
+    # grid's low face along that axis, and the cell index must stay below n
+    cx = np.minimum(cx, n - 1)
+    cy = np.minimum(cy, n - 1)
+    cz = np.minimum(cz, n - 1)
+    return (cx * n + cy) * n + cz
+
+
+def test_volume_512_full_size_properties():
+    import torch
+    from underwaterworld_amd import Perlin, build_volume
+    p = PR.preset("volume512")
+    n, L = 512, 513
+    perlin = Perlin(p)
+    vol, tape = build_volume(perlin, n, collision=True, tape=True)
+    _sync()
+    info = vol.info()[0]
+    V, I = int(info["n_verts"]), int(info["n_inds"])
+    T = I // 3
+    assert info["flags"] == 0 and V > 10_000_000 and I == 3 * T
+    act = int(info["n_active"]) / n ** 3
+    inds = vol.inds[:I].cpu().numpy().view(np.uint32)
+    tp = tape[:V].cpu().numpy()
+    a, b = tp[:, 0].view(np.uint32), tp[:, 1].view(np.uint32)
+    ra, rb = tp[:, 2].view(np.float32), tp[:, 3].view(np.float32)
+    verts = vol.verts[:V].cpu().numpy()
+    cells = vol.tri_cells[:T].cpu().numpy().view(np.uint32)
+    del vol, tape
+    torch.cuda.empty_cache()
+    # indices in range, every vertex used
+    assert int(inds.max()) == V - 1
+    used = np.zeros(V, bool)
+    used[inds] = True
+    assert used.all()
+    # each record once: ordered corner pairs unique; endpoints adjacent; densities straddle iso
+    key = a.astype(np.uint64) * np.uint64(L ** 3) + b.astype(np.uint64)
+    assert len(np.unique(key)) == V
+    d = np.abs(b.astype(np.int64) - a.astype(np.int64))
+    assert np.all((d == 1) | (d == L) | (d == L * L))
+    iso = np.float32(p["iso"])
+    assert np.all(((ra < iso) & (rb >= iso)) | ((rb < iso) & (ra >= iso)))
+    # vertices numbered in scan order of their owner cells (first-seen order)
+    own = _owner_cell_of_record(a, b, L, n)
+    assert np.all(np.diff(own) >= 0)
+    # triangles in cell order; each triangle's records are edges of its cell
+    assert np.all(np.diff(cells.astype(np.int64)) >= 0)
+    tri_v = inds.reshape(T, 3)
+    cx, cy, cz = cells // (n * n), (cells // n) % n, cells % n
+    for k in range(3):
+        va = a[tri_v[:, k]].astype(np.int64)
+        vb = b[tri_v[:, k]].astype(np.int64)
+        for s in (va, vb):
+            sx, sy, sz = s // (L * L), (s // L) % L, s % L
+            assert np.all((sx - cx >= 0) & (sx - cx <= 1) & (sy - cy >= 0) & (sy - cy <= 1) &
+                          (sz - cz >= 0) & (sz - cz <= 1))
+    # vertex positions: on the edge between the endpoints (world = sample index * step)
+    pa = np.stack([a // (L * L), (a // L) % L, a % L], 1).astype(np.float64)
+    pb = np.stack([b // (L * L), (b // L) % L, b % L], 1).astype(np.float64)
+    t = (verts.astype(np.float64) - pa) @ np.ones(3) / ((pb - pa) @ np.ones(3))
+    assert np.all((t >= -1e-6) & (t <= 1 + 1e-6))
+    # densities: a sample of tape entries against the oracle's iso_at at the sample coordinates
+    g = np.random.default_rng(5)
+    pick = g.choice(V, size=20000, replace=False)
+    freq = p["frequency"]
+    pts = np.concatenate([pa[pick], pb[pick]]) * freq
+    want = O.iso_at(p, pts)
+    got = np.concatenate([ra[pick], rb[pick]]).astype(np.float64)
+    err = float(np.max(np.abs(got - want)))
+    print(f"\nvolume 512^3: {V} vertices, {I} indices, active {act:.3f}, tape densities vs iso_at max {err:.3g}")
+    assert err <= DENS_TOL
+
+
+def _cases(d, n, iso):
+    D = d.reshape(n + 1, n + 1, n + 1) < np.float32(iso)
+    c = np.zeros((n, n, n), np.uint8)
+    for i, (a, b, e) in enumerate(CORNERS):
+        c |= (D[a:a + n, b:b + n, e:e + n].astype(np.uint8) << i)
+    return c.ravel()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+def _plane_grad(rec, target, oz=0.0):
+    import torch
+    nv = int(rec.batch.totals()["n_verts"])
+    v = rec.batch.verts[:nv].clone().requires_grad_(True)
+    loss = ((v[:, 2] - oz - target) ** 2).sum()
+    (g,) = torch.autograd.grad(loss, v)
+    return g
+
+
+def test_volume_adjoint_against_fp64_and_fd():
+    from underwaterworld_amd import Perlin, sample_density
+    from underwaterworld_amd.adjoint import VolumeMesher
+    n = 40
+    p = PR.preset("volume512", chunk_extent=n, resolution=n)
+    perlin = Perlin(p)
+    m = VolumeMesher(perlin, n, want_dens_grad=True)
+    rec = m.forward()
+    _sync()
+    target = 0.25 * n
+    g = m.backward(_plane_grad(rec, target), record=rec)
+    dens, _ = sample_density(perlin, [(0, 0, 0)], n)
+    _sync()
+    d = dens.cpu().numpy()[0]
+    ga, go, gd = g.grad_amp.cpu().numpy()[0], g.grad_off.cpu().numpy()[0], g.grad_dens.cpu().numpy()[0]
+    cs = _cases(d, n, p["iso"])
+    r = O.adjoint_chunk(p, (0, 0, 0), n, target, cases=cs, rho=d.astype(np.float64))
+    assert r["nverts"] == rec.batch.info()[0]["n_verts"]
+    worst = {"amp": _rel(ga, r["g_amp"]), "off": _rel(go, r["g_off"]), "dens": _rel(gd, r["g_dens"])}
+    h = 1e-3
+
+    def f_at(amp=None, ax=None):
+        def f(dl):
+            q = dict(p, amplitudes=list(p["amplitudes"]))
+            if amp is not None:
+                q["amplitudes"][amp] += dl
+            if ax is not None:
+                o = [0.0, 0.0, 0.0]
+                o[ax] = dl
+                q["offset"] = tuple(o)
+            return O.adjoint_chunk(q, (0, 0, 0), n, target, cases=cs, grads=False)["loss"]
+        return f
+
+    def rich(f):
+        return (4 * ((f(h / 2) - f(-h / 2)) / h) - (f(h) - f(-h)) / (2 * h)) / 3
+
+    fd = np.array([rich(f_at(amp=j)) for j in (0, 2, 5, 7)] + [rich(f_at(ax=k)) for k in range(3)])
+    gk = np.concatenate([ga[[0, 2, 5, 7]], go])
+    worst["fd"] = float(np.max(np.abs(gk - fd) / np.maximum(np.abs(fd), 1e-3 * np.abs(fd).max())))
+    print(f"\nvolume adjoint n={n}: {worst}")
+    assert all(v <= 1e-4 for v in worst.values())
+
+
+def test_volume_adjoint_equals_batched_adjoint():
+    import torch
+    from underwaterworld_amd import Perlin
+    from underwaterworld_amd.adjoint import ChunkMesher, VolumeMesher
+    p = PR.preset("volume_bricks")
+    coord = (3, 5, 0)
+    pv, pb = Perlin(p), Perlin(p)
+    vm = VolumeMesher(pv, 32, coord, want_dens_grad=True)
+    bm = ChunkMesher(pb, [coord], 32, want_dens_grad=True)
+    rv, rb = vm.forward(), bm.forward()
+    _sync()
+    oz = float(p["chunk_extent"] * coord[2])
+    gv = vm.backward(_plane_grad(rv, 8.0, oz), record=rv)
+    gb = bm.backward(_plane_grad(rb, 8.0, oz), record=rb)
+    torch.cuda.synchronize()
+    assert np.array_equal(rv.batch.verts[: int(rv.batch.info()[0]["n_verts"])].cpu().numpy(),
+                          rb.batch.verts[: int(rb.batch.info()[0]["n_verts"])].cpu().numpy())
+    assert np.allclose(gv.grad_amp.cpu().numpy(), gb.grad_amp.cpu().numpy(), rtol=1e-11, atol=0)
+    assert np.allclose(gv.grad_off.cpu().numpy(), gb.grad_off.cpu().numpy(), rtol=1e-11, atol=0)
+    assert np.allclose(gv.grad_dens.cpu().numpy()[0], gb.grad_dens.cpu().numpy()[0][: 33 ** 3], rtol=1e-5,
+                       atol=1e-6)
+    _ = TIE_TOL

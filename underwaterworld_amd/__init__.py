@@ -6,9 +6,9 @@ interface (noise::Perlin + Chunk) and the device-resident multi-chunk forms.
 """
 from . import params
 from .engine import (BatchedGraph, Chunk, MeshBatch, Perlin, Tri, batch_capacity, build_chunks_batched,
-                     build_chunks_plain, density_at, get_batch_tuning, permutation_table, sample_density,
-                     set_batch_tuning, tri_ids_around)
+                     build_chunks_plain, build_volume, density_at, get_batch_tuning, permutation_table,
+                     sample_density, set_batch_tuning, tri_ids_around, volume_required_sizes)
 
 __all__ = ["params", "Perlin", "Chunk", "Tri", "MeshBatch", "BatchedGraph", "build_chunks_plain",
            "build_chunks_batched", "batch_capacity", "sample_density", "density_at", "permutation_table",
-           "get_batch_tuning", "set_batch_tuning", "tri_ids_around"]
+           "get_batch_tuning", "set_batch_tuning", "tri_ids_around", "build_volume", "volume_required_sizes"]

@@ -58,6 +58,7 @@ class UwTotals(ctypes.Structure):
 
 
 BATCH_MAX_RESOLUTION = 32
+VOLUME_MAX_RESOLUTION = 640
 CHUNK_INVALID = 8
 
 
@@ -131,6 +132,11 @@ SIGNATURES = {
     "uw_adjoint_backward": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, c_void_p, c_void_p,
                                            c_void_p, c_void_p, ctypes.c_int64, c_void_p, c_void_p, c_void_p]),
     "uw_graph_destroy": (None, [c_void_p]),
+    "uw_build_volume": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                       P(UwMeshOut), c_void_p, c_void_p]),
+    "uw_volume_backward": (ctypes.c_int, [c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "uw_volume_scratch_bytes": (ctypes.c_uint64, [c_void_p, ctypes.c_int32]),
     "uw_world_config_reference": (None, [P(UwWorldConfig)]),
     "uw_world_plan": (ctypes.c_int, [P(UwWorldConfig), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                                      ctypes.c_int32, c_void_p, ctypes.c_int32, P(ctypes.c_int32), P(UwWorldCandidate),

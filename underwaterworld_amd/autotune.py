@@ -168,7 +168,7 @@ def main():
     jobs = [("chunks12", params.grid_coords(-8, 8), [12] * 4096)]
     mc, mr = params.mixed_workload(8192, 7)
     jobs.append(("mixed", mc, mr))
-    jobs.append(("volume512", params.volume_bricks(16), [32] * 4096))
+    jobs.append(("volume_bricks", params.volume_bricks(16), [32] * 4096))
     for name, coords, res in jobs:
         p = params.preset(name)
         perlin = Perlin(p, apply_tuning=False)

@@ -25,52 +25,6 @@ namespace {
 constexpr int kMaxL = UW_BATCH_MAX_RESOLUTION + 1;
 constexpr int kBlock = 256;
 
-// Octave value and gradient with respect to the octave coordinate (f32), given the per-axis
-// lattice data and fade derivatives s'(d) = 30 d^2 (d - 1)^2.
-__device__ __forceinline__ float uw_octave_grad(const uw_axis& X, const uw_axis& Y, const uw_axis& Z, float dfx,
-                                                float dfy, float dfz, const uint8_t* P, const float* G3,
-                                                float g[3]) {
-    const uint32_t a0 = X.bits & 255, a1 = X.bits >> 8;
-    const uint32_t y0 = Y.bits & 255, y1 = Y.bits >> 8;
-    const uint32_t z0 = Z.bits & 255, z1 = Z.bits >> 8;
-    const uint32_t b00 = P[a0 ^ y0], b01 = P[a0 ^ y1], b10 = P[a1 ^ y0], b11 = P[a1 ^ y1];
-    const float4 G000 = uw_grad3(G3, b00 ^ z0), G100 = uw_grad3(G3, b10 ^ z0), G010 = uw_grad3(G3, b01 ^ z0),
-                 G110 = uw_grad3(G3, b11 ^ z0);
-    const float4 G001 = uw_grad3(G3, b00 ^ z1), G101 = uw_grad3(G3, b10 ^ z1), G011 = uw_grad3(G3, b01 ^ z1),
-                 G111 = uw_grad3(G3, b11 ^ z1);
-    const float g000 = uw_gdot(G000, X.d, Y.d, Z.d), g100 = uw_gdot(G100, X.dm1, Y.d, Z.d);
-    const float g010 = uw_gdot(G010, X.d, Y.dm1, Z.d), g110 = uw_gdot(G110, X.dm1, Y.dm1, Z.d);
-    const float g001 = uw_gdot(G001, X.d, Y.d, Z.dm1), g101 = uw_gdot(G101, X.dm1, Y.d, Z.dm1);
-    const float g011 = uw_gdot(G011, X.d, Y.dm1, Z.dm1), g111 = uw_gdot(G111, X.dm1, Y.dm1, Z.dm1);
-    const float a = X.fade, b = Y.fade, c = Z.fade;
-    const float x00 = uw_lerp(a, g000, g100), x10 = uw_lerp(a, g010, g110);
-    const float x01 = uw_lerp(a, g001, g101), x11 = uw_lerp(a, g011, g111);
-    const float yv0 = uw_lerp(b, x00, x10), yv1 = uw_lerp(b, x01, x11);
-    float r = uw_lerp(c, yv0, yv1) * UW_PERLIN_SCALE;
-    // d/dx
-    const float dx00 = fmaf(dfx, g100 - g000, uw_lerp(a, G000.x, G100.x));
-    const float dx10 = fmaf(dfx, g110 - g010, uw_lerp(a, G010.x, G110.x));
-    const float dx01 = fmaf(dfx, g101 - g001, uw_lerp(a, G001.x, G101.x));
-    const float dx11 = fmaf(dfx, g111 - g011, uw_lerp(a, G011.x, G111.x));
-    const float drx = uw_lerp(c, uw_lerp(b, dx00, dx10), uw_lerp(b, dx01, dx11));
-    // d/dy
-    const float dy0 = fmaf(dfy, x10 - x00, uw_lerp(b, uw_lerp(a, G000.y, G100.y), uw_lerp(a, G010.y, G110.y)));
-    const float dy1 = fmaf(dfy, x11 - x01, uw_lerp(b, uw_lerp(a, G001.y, G101.y), uw_lerp(a, G011.y, G111.y)));
-    const float dry = uw_lerp(c, dy0, dy1);
-    // d/dz
-    const float ez0 = uw_lerp(b, uw_lerp(a, G000.z, G100.z), uw_lerp(a, G010.z, G110.z));
-    const float ez1 = uw_lerp(b, uw_lerp(a, G001.z, G101.z), uw_lerp(a, G011.z, G111.z));
-    const float drz = fmaf(dfz, yv1 - yv0, uw_lerp(c, ez0, ez1));
-    if (r > 1.0f || r < -1.0f) {
-        g[0] = g[1] = g[2] = 0.0f;
-        return r > 1.0f ? 1.0f : -1.0f;
-    }
-    g[0] = drx * UW_PERLIN_SCALE;
-    g[1] = dry * UW_PERLIN_SCALE;
-    g[2] = drz * UW_PERLIN_SCALE;
-    return r;
-}
-
 struct BackArgs {
     uw_field f;
     const uw_tables* tab;
@@ -314,6 +268,7 @@ int uw_adjoint_backward(uw_ctx* ctx, const int32_t* coords, const int32_t* res, 
     a.grad_amp = grad_amp;
     a.grad_off = grad_off;
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int rc = uw_refuse_capture(ctx, st)) return rc;
     const int O = a.f.octaves;
     if (int rc = uw_prof_start(ctx, st)) return rc;
     int lrc = UW_OK;

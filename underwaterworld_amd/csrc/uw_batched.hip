@@ -1104,6 +1104,7 @@ int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     if (!args_ok(nchunks, coords, res, out)) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
     UW_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int rc = uw_refuse_capture(ctx, st)) return rc;
     BatchLaunch bl;
     if (int rc = prepare_batched(ctx, nchunks > 0 ? nchunks : 1, &bl)) return rc;
     if (nchunks == 0) {
@@ -1122,6 +1123,7 @@ int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32
         return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
     UW_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int rc = uw_refuse_capture(ctx, st)) return rc;
     BatchLaunch bl;
     if (int rc = prepare_batched(ctx, nchunks > 0 ? nchunks : 1, &bl)) return rc;
     if (nchunks == 0) {

@@ -16,6 +16,17 @@ int uw_hip_fail(uw_ctx* ctx, hipError_t e, const char* what) {
     return uw_fail(ctx, e == hipErrorOutOfMemory ? UW_ERR_OOM : UW_ERR_HIP, m);
 }
 
+int uw_refuse_capture(uw_ctx* ctx, hipStream_t st) {
+    if (!st) return UW_OK;  // the null stream cannot be captured
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    UW_HIP(ctx, hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return uw_fail(ctx, UW_ERR_INVALID_ARG,
+                       "stream is capturing: this entry point uses context-owned scratch that a later call may "
+                       "reallocate; capture the batched build with uw_batched_graph_create");
+    return UW_OK;
+}
+
 int uw_prof_start(uw_ctx* ctx, hipStream_t st) {
     if (!ctx->profiling) return UW_OK;
     if (ctx->prof_used + 2 > ctx->prof_events.size()) {
@@ -232,6 +243,7 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(ctx->d_batch_order);
     (void)hipFree(ctx->d_batch_ztab);
     (void)hipFree(ctx->d_batch_dscr);
+    (void)hipFree(ctx->d_vol);
     (void)hipFree(ctx->d_adj_part);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
     delete ctx;

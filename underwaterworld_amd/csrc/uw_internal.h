@@ -65,6 +65,8 @@ struct uw_ctx {
     size_t batch_dscr_bytes = 0;
     double* d_adj_part = nullptr;  // adjoint backward: per-(chunk, vertex share) parameter sums
     size_t adj_part_bytes = 0;
+    unsigned char* d_vol = nullptr;  // single-volume path: tables, density field, row counts, partial sums
+    size_t vol_bytes = 0;
     uint8_t perm[256];
     uw_tables host_tables;
     uw_tables* d_tables = nullptr;
@@ -86,6 +88,9 @@ struct uw_ctx {
 };
 
 int uw_fail(uw_ctx* ctx, int code, const std::string& msg);
+// UW_ERR_INVALID_ARG when `st` is capturing a graph: entry points whose launches bind
+// context-owned scratch (reallocated by later calls that need more) must not be captured.
+int uw_refuse_capture(uw_ctx* ctx, hipStream_t st);
 // Record the start / end event of a timed kernel on `st` (no-ops unless profiling is enabled).
 int uw_prof_start(uw_ctx* ctx, hipStream_t st);
 int uw_prof_end(uw_ctx* ctx, hipStream_t st);

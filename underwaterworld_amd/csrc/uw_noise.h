@@ -356,6 +356,85 @@ struct uw_octaves {
     __host__ __device__ static constexpr bool accepts(int32_t o) { return OCT > 0 ? o == OCT : o <= UW_MAX_OCTAVES; }
 };
 
+// Face-flag word of a z record that sets up every octave's lattice cell from scratch (bit 2k per
+// octave, no "low face is the previous high face" bits): the first sample of a line that starts
+// inside the z range.
+__device__ __forceinline__ uint32_t uw_full_setup_flags(int32_t octaves) {
+    return 0x55555555u & ((octaves >= 16) ? 0xFFFFFFFFu : ((1u << (2 * octaves)) - 1u));
+}
+
+// One z-line of samples: the line whose octave-k x / y axis entries are AX(k) / AY(k) (hashed x),
+// z records from zr on (rw words each; the record after the last sample is fetched ahead and
+// discarded), `count` samples. FORCE: the first sample builds every octave's lattice cell whatever
+// its record's face flags say (a line that starts inside a longer z range). store(iz, rho)
+// receives the densities; vmin / vmax keep the running extremes.
+template <int OCT, bool FORCE, class AxisX, class AxisY, class Store>
+__device__ __forceinline__ void uw_noise_zline(const uw_field& f, AxisX AX, AxisY AY, const uint8_t* P,
+                                               const float* G3, const uint32_t* zr, int32_t rw, int32_t count,
+                                               float& vmin, float& vmax, Store store) {
+    constexpr int OMAX = uw_octaves<OCT>::kMax;
+    const int32_t O = uw_octaves<OCT>::count(f);
+    uint32_t bb[OMAX];
+    uw_line_oct st[OMAX];
+#pragma unroll
+    for (int k = 0; k < OMAX; ++k) {
+        if (k < O) {
+            const uint32_t xb2 = AX(k).bits, yb = AY(k).bits;
+            const uint32_t a0 = xb2 & 255u, a1 = xb2 >> 8, y0 = yb & 255u, y1 = yb >> 8;
+            bb[k] = uint32_t(P[a0 ^ y0]) | (uint32_t(P[a0 ^ y1]) << 8) | (uint32_t(P[a1 ^ y0]) << 16) |
+                    (uint32_t(P[a1 ^ y1]) << 24);
+        }
+    }
+    // Sample iz, then the fetch of the record of sample iz + 1 (the fetch after the last sample
+    // reads the spare record and is discarded). The fetch is issued after the face updates,
+    // whose LDS reads would otherwise wait for it too, and before the sample arithmetic, which
+    // covers its latency. Two records alternate, so the fetched registers are used in place.
+    auto sample = [&](const uw_zsample<OMAX>& Z, uw_zsample<OMAX>& Znext, int32_t iz) {
+        if (Z.flags) {  // some octave enters a new lattice cell
+#pragma unroll
+            for (int k = 0; k < OMAX; ++k) {
+                if (k < O && (Z.flags & (1u << (2 * k)))) {
+                    const uw_axis X = AX(k), Y = AY(k);
+                    const uint32_t zb = Z.bits[k];
+                    if (Z.flags & (2u << (2 * k))) {
+                        st[k].A0 = st[k].A1;
+                        st[k].B0 = st[k].B1;
+                        uw_line_face(X, Y, bb[k], zb >> 8, G3, &st[k].A1, &st[k].B1);
+                        uw_line_diff(st[k]);
+                    } else {
+                        st[k] = uw_line_setup(X, Y, bb[k], zb, G3);
+                    }
+                }
+            }
+        }
+        zr += rw;
+        Znext.load(zr, O);
+        __builtin_amdgcn_sched_barrier(0);  // keep the fetch ahead of the arithmetic
+        float total = 0.0f;
+#pragma unroll
+        for (int k = 0; k < OMAX; ++k)
+            if (k < O) total = fmaf(f.weight[k], uw_line_sample(st[k], Z.d[k], Z.fade[k], f.clamp_lo, f.clamp_hi),
+                                    total);
+        const float rho = (Z.adj + total) + Z.post;
+        store(iz, rho);
+        vmin = uw_fmin_raw(vmin, rho);
+        vmax = uw_fmax_raw(vmax, rho);
+    };
+    uw_zsample<OMAX> Za, Zb;
+    Za.load(zr, O);
+    if (FORCE) Za.flags = uw_full_setup_flags(O);
+    int32_t iz = 0;
+    for (; iz + 1 < count; iz += 2) {  // no exit between a fetch and its use
+        sample(Za, Zb, iz);
+        sample(Zb, Za, iz + 1);
+    }
+    if (iz < count) sample(Za, Zb, iz);
+}
+
+__device__ __forceinline__ int uw_below_bits(float vmin, float vmax, float iso) {
+    return (vmin <= iso ? UW_BELOW_NOT_ABOVE : 0) | (vmin < iso ? UW_BELOW_LT : 0) | (vmax >= iso ? UW_BELOW_GE : 0);
+}
+
 // Densities of x-planes [xa, xb) of a chunk whose tables are in T, one thread per (x, y) line.
 // Plane x is written to dst + slot(x) * L * L with slot(x) = x % nslot, z fastest.
 // Returns, per thread, the OR of the UW_BELOW_* bits of its samples.
@@ -363,8 +442,6 @@ template <int OCT>
 __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uw_ztab& Zt,
                                                const uint8_t* P, const float* G3, int32_t L, int32_t xa, int32_t xb,
                                                float* dst, int32_t nslot, int tid, int nthr) {
-    constexpr int OMAX = uw_octaves<OCT>::kMax;
-    const int32_t O = uw_octaves<OCT>::count(f);
     const int32_t lines = (xb - xa) * L;
     const uw_divider dL = uw_make_divider(uint32_t(L)), dslot = uw_make_divider(uint32_t(nslot));
     const int32_t rw = OCT > 0 ? uw_zrec_words(OCT) : Zt.rw;
@@ -374,62 +451,55 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
         const int32_t tq = int32_t(uw_div(uint32_t(t), dL));
         const int32_t ix = xa + tq, iy = t - tq * L;
         float* out = dst + int32_t(uw_mod(uint32_t(ix), dslot)) * L * L + iy * L;
-        uint32_t bb[OMAX];
-        uw_line_oct st[OMAX];
-#pragma unroll
-        for (int k = 0; k < OMAX; ++k) {
-            if (k < O) {
-                const uint32_t xb2 = T.AX[k * L + ix].bits, yb = T.AY[k * L + iy].bits;
-                const uint32_t a0 = xb2 & 255u, a1 = xb2 >> 8, y0 = yb & 255u, y1 = yb >> 8;
-                bb[k] = uint32_t(P[a0 ^ y0]) | (uint32_t(P[a0 ^ y1]) << 8) | (uint32_t(P[a1 ^ y0]) << 16) |
-                        (uint32_t(P[a1 ^ y1]) << 24);
-            }
-        }
-        // Sample iz, then the fetch of the record of sample iz + 1 (the fetch after the last sample
-        // reads the spare record and is discarded). The fetch is issued after the face updates,
-        // whose LDS reads would otherwise wait for it too, and before the sample arithmetic, which
-        // covers its latency. Two records alternate, so the fetched registers are used in place.
-        const uint32_t* zr = Zt.rec;  // record of the sample being evaluated
-        auto sample = [&](const uw_zsample<OMAX>& Z, uw_zsample<OMAX>& Znext, int32_t iz) {
-            if (Z.flags) {  // some octave enters a new lattice cell
-#pragma unroll
-                for (int k = 0; k < OMAX; ++k) {
-                    if (k < O && (Z.flags & (1u << (2 * k)))) {
-                        const uw_axis X = T.AX[k * L + ix], Y = T.AY[k * L + iy];
-                        const uint32_t zb = Z.bits[k];
-                        if (Z.flags & (2u << (2 * k))) {
-                            st[k].A0 = st[k].A1;
-                            st[k].B0 = st[k].B1;
-                            uw_line_face(X, Y, bb[k], zb >> 8, G3, &st[k].A1, &st[k].B1);
-                            uw_line_diff(st[k]);
-                        } else {
-                            st[k] = uw_line_setup(X, Y, bb[k], zb, G3);
-                        }
-                    }
-                }
-            }
-            zr += rw;
-            Znext.load(zr, O);
-            __builtin_amdgcn_sched_barrier(0);  // keep the fetch ahead of the arithmetic
-            float total = 0.0f;
-#pragma unroll
-            for (int k = 0; k < OMAX; ++k)
-                if (k < O) total = fmaf(f.weight[k], uw_line_sample(st[k], Z.d[k], Z.fade[k], f.clamp_lo, f.clamp_hi),
-                                        total);
-            const float rho = (Z.adj + total) + Z.post;
-            out[iz] = rho;
-            vmin = uw_fmin_raw(vmin, rho);
-            vmax = uw_fmax_raw(vmax, rho);
-        };
-        uw_zsample<OMAX> Za, Zb;
-        Za.load(zr, O);
-        int32_t iz = 0;
-        for (; iz + 1 < L; iz += 2) {  // no exit between a fetch and its use
-            sample(Za, Zb, iz);
-            sample(Zb, Za, iz + 1);
-        }
-        if (iz < L) sample(Za, Zb, iz);
+        uw_noise_zline<OCT, false>(
+            f, [&](int k) { return T.AX[k * L + ix]; }, [&](int k) { return T.AY[k * L + iy]; }, P, G3, Zt.rec, rw, L,
+            vmin, vmax, [&](int32_t iz, float rho) { out[iz] = rho; });
     }
-    return (vmin <= f.iso ? UW_BELOW_NOT_ABOVE : 0) | (vmin < f.iso ? UW_BELOW_LT : 0) |
-           (vmax >= f.iso ? UW_BELOW_GE : 0);
+    return uw_below_bits(vmin, vmax, f.iso);
+}
+
+// Octave value and gradient with respect to the octave coordinate (f32), given the per-axis
+// lattice data and fade derivatives s'(d) = 30 d^2 (d - 1)^2.
+__device__ __forceinline__ float uw_octave_grad(const uw_axis& X, const uw_axis& Y, const uw_axis& Z, float dfx,
+                                                float dfy, float dfz, const uint8_t* P, const float* G3,
+                                                float g[3]) {
+    const uint32_t a0 = X.bits & 255, a1 = X.bits >> 8;
+    const uint32_t y0 = Y.bits & 255, y1 = Y.bits >> 8;
+    const uint32_t z0 = Z.bits & 255, z1 = Z.bits >> 8;
+    const uint32_t b00 = P[a0 ^ y0], b01 = P[a0 ^ y1], b10 = P[a1 ^ y0], b11 = P[a1 ^ y1];
+    const float4 G000 = uw_grad3(G3, b00 ^ z0), G100 = uw_grad3(G3, b10 ^ z0), G010 = uw_grad3(G3, b01 ^ z0),
+                 G110 = uw_grad3(G3, b11 ^ z0);
+    const float4 G001 = uw_grad3(G3, b00 ^ z1), G101 = uw_grad3(G3, b10 ^ z1), G011 = uw_grad3(G3, b01 ^ z1),
+                 G111 = uw_grad3(G3, b11 ^ z1);
+    const float g000 = uw_gdot(G000, X.d, Y.d, Z.d), g100 = uw_gdot(G100, X.dm1, Y.d, Z.d);
+    const float g010 = uw_gdot(G010, X.d, Y.dm1, Z.d), g110 = uw_gdot(G110, X.dm1, Y.dm1, Z.d);
+    const float g001 = uw_gdot(G001, X.d, Y.d, Z.dm1), g101 = uw_gdot(G101, X.dm1, Y.d, Z.dm1);
+    const float g011 = uw_gdot(G011, X.d, Y.dm1, Z.dm1), g111 = uw_gdot(G111, X.dm1, Y.dm1, Z.dm1);
+    const float a = X.fade, b = Y.fade, c = Z.fade;
+    const float x00 = uw_lerp(a, g000, g100), x10 = uw_lerp(a, g010, g110);
+    const float x01 = uw_lerp(a, g001, g101), x11 = uw_lerp(a, g011, g111);
+    const float yv0 = uw_lerp(b, x00, x10), yv1 = uw_lerp(b, x01, x11);
+    float r = uw_lerp(c, yv0, yv1) * UW_PERLIN_SCALE;
+    // d/dx
+    const float dx00 = fmaf(dfx, g100 - g000, uw_lerp(a, G000.x, G100.x));
+    const float dx10 = fmaf(dfx, g110 - g010, uw_lerp(a, G010.x, G110.x));
+    const float dx01 = fmaf(dfx, g101 - g001, uw_lerp(a, G001.x, G101.x));
+    const float dx11 = fmaf(dfx, g111 - g011, uw_lerp(a, G011.x, G111.x));
+    const float drx = uw_lerp(c, uw_lerp(b, dx00, dx10), uw_lerp(b, dx01, dx11));
+    // d/dy
+    const float dy0 = fmaf(dfy, x10 - x00, uw_lerp(b, uw_lerp(a, G000.y, G100.y), uw_lerp(a, G010.y, G110.y)));
+    const float dy1 = fmaf(dfy, x11 - x01, uw_lerp(b, uw_lerp(a, G001.y, G101.y), uw_lerp(a, G011.y, G111.y)));
+    const float dry = uw_lerp(c, dy0, dy1);
+    // d/dz
+    const float ez0 = uw_lerp(b, uw_lerp(a, G000.z, G100.z), uw_lerp(a, G010.z, G110.z));
+    const float ez1 = uw_lerp(b, uw_lerp(a, G001.z, G101.z), uw_lerp(a, G011.z, G111.z));
+    const float drz = fmaf(dfz, yv1 - yv0, uw_lerp(c, ez0, ez1));
+    if (r > 1.0f || r < -1.0f) {
+        g[0] = g[1] = g[2] = 0.0f;
+        return r > 1.0f ? 1.0f : -1.0f;
+    }
+    g[0] = drx * UW_PERLIN_SCALE;
+    g[1] = dry * UW_PERLIN_SCALE;
+    g[2] = drz * UW_PERLIN_SCALE;
+    return r;
 }

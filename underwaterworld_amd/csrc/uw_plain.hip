@@ -406,6 +406,7 @@ int uw_sample_density(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, int32
     if (nchunks < 0 || (nchunks > 0 && (!coords || !dens))) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
     if (nchunks == 0) return UW_OK;
     UW_HIP(ctx, hipSetDevice(ctx->device));
+    if (int rc = uw_refuse_capture(ctx, static_cast<hipStream_t>(stream))) return rc;
     const uw_field f = uw_make_field(ctx->params);
     uint32_t* flags = below;
     if (!flags) {
@@ -425,6 +426,7 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
         return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
     UW_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (int rc = uw_refuse_capture(ctx, st)) return rc;
     if (nchunks == 0) {
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t*>(out->totals),
                            uint64_t(sizeof(uw_totals) / 4), 0u);

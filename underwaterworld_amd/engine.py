@@ -66,6 +66,29 @@ class Perlin:
     def ctx(self):
         return self._ctx
 
+    def field(self):
+        """(amplitudes float64 (octaves,), offset float64 (3,)) of the context's current field."""
+        p = self.params
+        return (np.array(p["amplitudes"][: int(p["octaves"])], np.float64),
+                np.array(p.get("offset", (0.0, 0.0, 0.0)), np.float64))
+
+    def set_field(self, amplitudes=None, offset=None):
+        """Replace the octave amplitudes and/or the world-space sample offset (uw_ctx_set_field);
+        None leaves a part unchanged. Later builds on this context use the new field."""
+        a = None if amplitudes is None else np.ascontiguousarray(amplitudes, np.float64).ravel()
+        o = None if offset is None else np.ascontiguousarray(offset, np.float64).ravel()
+        if a is not None and len(a) != int(self.params["octaves"]):
+            raise ValueError(f"expected {self.params['octaves']} amplitudes, got {len(a)}")
+        if o is not None and len(o) != 3:
+            raise ValueError("offset needs 3 values")
+        rc = _lib.lib().uw_ctx_set_field(self._ctx, a.ctypes.data_as(ctypes.c_void_p) if a is not None else None,
+                                         o.ctypes.data_as(ctypes.c_void_p) if o is not None else None)
+        _lib.check(rc, self._ctx)
+        if a is not None:
+            self.params["amplitudes"] = [float(v) for v in a]
+        if o is not None:
+            self.params["offset"] = tuple(float(v) for v in o)
+
     def permutation(self) -> np.ndarray:
         out = np.zeros(256, np.uint8)
         _lib.check(_lib.lib().uw_ctx_permutation(self._ctx, out.ctypes.data_as(_lib.P(ctypes.c_uint8))), self._ctx)
@@ -405,6 +428,44 @@ class BatchedGraph:
             self.close()
         except Exception:
             pass
+
+
+def build_volume(perlin: Perlin, n: int, chunk=(0, 0, 0), vert_cap: int | None = None, ind_cap: int | None = None,
+                 stream=None, colors: bool = False, u16: bool = False, collision: bool = False,
+                 tape: bool = False):
+    """One chunk of n cells per axis (up to 640) meshed as one grid with one index space
+    (uw_build_volume): Chunk::new(chunk) + build_full at resolution n. Capacities default to the
+    exact sizes, found by a counting build first (one host synchronisation). Returns the MeshBatch
+    (one info record) and, with tape=True, the (vert_cap, 4) int32 vertex tape for the backward."""
+    import torch
+    dev = torch.device("cuda", perlin.device)
+    if vert_cap is None or ind_cap is None:
+        v, i = volume_required_sizes(perlin, n, chunk)
+        vert_cap = v if vert_cap is None else vert_cap
+        ind_cap = i if ind_cap is None else ind_cap
+    verts, inds, info, totals = _alloc_out(1, vert_cap, ind_cap, dev, colors, u16)
+    coll = _alloc_collision(ind_cap, dev) if collision else None
+    tp = torch.empty((max(vert_cap, 1), 4), dtype=torch.int32, device=dev) if tape else None
+    out = _mesh_out(verts, inds, info, totals, vert_cap, ind_cap, coll)
+    rc = _lib.lib().uw_build_volume(perlin.ctx, int(chunk[0]), int(chunk[1]), int(chunk[2]), int(n), ctypes.byref(out),
+                                    ctypes.c_void_p(tp.data_ptr()) if tp is not None else None, _stream_ptr(stream))
+    _lib.check(rc, perlin.ctx)
+    b = MeshBatch(verts, inds, info[:32], totals, None, *(coll or (None, None)))
+    return (b, tp) if tape else b
+
+
+def volume_required_sizes(perlin: Perlin, n: int, chunk=(0, 0, 0)) -> tuple[int, int]:
+    """(vertices, indices) of the volume: a build with zero capacity reports the totals without
+    writing a mesh."""
+    import torch
+    dev = torch.device("cuda", perlin.device)
+    verts, inds, info, totals = _alloc_out(1, 0, 0, dev)
+    out = _mesh_out(verts, inds, info, totals, 0, 0)
+    rc = _lib.lib().uw_build_volume(perlin.ctx, int(chunk[0]), int(chunk[1]), int(chunk[2]), int(n), ctypes.byref(out),
+                                    None, _stream_ptr(None))
+    _lib.check(rc, perlin.ctx)
+    t = MeshBatch(verts, inds, info, totals, None).totals()
+    return int(t["n_verts"]), int(t["n_inds"])
 
 
 def get_batch_tuning(perlin: Perlin) -> dict:

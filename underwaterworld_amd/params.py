@@ -55,12 +55,16 @@ MIXED = dict(CHUNKS_12, octaves=6, amplitudes=_amps(6))
 
 ADJOINT_24 = dict(CHUNKS_12, octaves=6, amplitudes=_amps(6), resolution=24)
 
-# One 512^3-cell volume (513^3 samples, one world unit per voxel, base frequency 1/32 per voxel),
-# built as 16^3 bricks of 32^3 cells whose sample coordinates coincide with the volume's. The
-# nearly flat bias (slope 0.01, offset -0.05) leaves about 15 % of the cells active (column-sampled
-# calibration of seed 1234).
-VOLUME_512 = dict(CHUNKS_12, seed=1234, octaves=8, amplitudes=_amps(8), chunk_extent=32, resolution=32,
+# One 512^3-cell volume (513^3 samples, one world unit per voxel, base frequency 1/32 per voxel):
+# a single chunk of extent 512 and resolution 512 at chunk coordinate (0, 0, 0), meshed as one grid
+# with one index space. The nearly flat bias (slope 0.01, offset -0.05) leaves about 15 % of the
+# cells active (column-sampled calibration of seed 1234).
+VOLUME_512 = dict(CHUNKS_12, seed=1234, octaves=8, amplitudes=_amps(8), chunk_extent=512, resolution=512,
                   bias=(0.01, -0.05, 0.0))
+
+# The same field cut into 16^3 independent chunks ("bricks") of 32^3 cells whose sample coordinates
+# coincide with the volume's (brick c covers samples 32 c .. 32 c + 32 per axis).
+VOLUME_BRICKS = dict(VOLUME_512, chunk_extent=32, resolution=32)
 
 
 def volume_bricks(bricks_per_axis: int = 16):
@@ -71,7 +75,7 @@ def volume_bricks(bricks_per_axis: int = 16):
 
 def preset(name: str, **overrides) -> dict:
     base = {"reference": REFERENCE, "chunks12": CHUNKS_12, "mixed": MIXED, "adjoint24": ADJOINT_24,
-            "volume512": VOLUME_512}[name]
+            "volume512": VOLUME_512, "volume_bricks": VOLUME_BRICKS}[name]
     p = copy.deepcopy(base)
     p.update(overrides)
     if "octaves" in overrides and "amplitudes" not in overrides:
