@@ -142,12 +142,28 @@ int uw_ctx_permutation(const uw_ctx* ctx, uint8_t* out256);
 /* Host-only: the permutation table noise::Perlin::new(seed) builds (no device needed). */
 void uw_permutation_table(uint32_t seed, uint8_t* out256);
 
-/* Kernel timing: while enabled, every build records HIP events on its stream around its dominant
- * kernel (the density kernel of the plain path, the persistent kernel of the batched path).
- * uw_profile_read synchronises on the recorded events, returns the summed milliseconds and the
- * number of timed launches since the last read, and resets both. Not for use under graph capture. */
+/* Kernel timing: while enabled, every build records HIP events on its stream around its kernels,
+ * each pair tagged with a stage. Stage 0 is the path's dominant kernel: the density kernel of the
+ * plain path, the persistent kernel of the batched path, the density kernel of the volume path and
+ * the backward kernel(s) of the adjoint. uw_profile_read synchronises on the recorded events,
+ * returns the summed milliseconds and the number of timed launches of stage 0 since the last read,
+ * and resets every stage; uw_profile_read_stages does the same for stages [0, nstages) into
+ * arrays. Not for use under graph capture. */
+enum uw_profile_stage {
+    UW_STAGE_DOMINANT = 0,
+    UW_STAGE_PLAIN_CLASSIFY = 1, /* k_classify: cases, counts, compaction */
+    UW_STAGE_PLAIN_OFFSETS = 2,  /* k_offsets: chunk offsets */
+    UW_STAGE_PLAIN_EMIT = 3,     /* k_emit: vertices and indices */
+    UW_STAGE_BATCH_SETUP = 4,    /* k_prep + k_place: z tables and queue order */
+    UW_STAGE_VOLUME_COUNT = 5,   /* k_vol_count */
+    UW_STAGE_VOLUME_SCAN = 6,    /* k_vol_scan_* */
+    UW_STAGE_VOLUME_EMIT = 7,    /* k_vol_emit */
+    UW_STAGE_VOLUME_TABLES = 8,  /* k_vol_tables */
+    UW_STAGE_COUNT = 9
+};
 int uw_profile_enable(uw_ctx* ctx, int on);
 int uw_profile_read(uw_ctx* ctx, double* total_ms, uint64_t* launches);
+int uw_profile_read_stages(uw_ctx* ctx, int32_t nstages, double* total_ms, uint64_t* launches);
 
 /* Upper bounds for one chunk of n cells per axis: vertices 5 n (n+1)^2, indices 15 n^3. */
 uint64_t uw_max_verts(int32_t n);

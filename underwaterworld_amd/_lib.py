@@ -59,6 +59,9 @@ class UwTotals(ctypes.Structure):
 
 BATCH_MAX_RESOLUTION = 32
 VOLUME_MAX_RESOLUTION = 640
+# enum uw_profile_stage
+STAGE_DOMINANT, STAGE_PLAIN_CLASSIFY, STAGE_PLAIN_OFFSETS, STAGE_PLAIN_EMIT, STAGE_BATCH_SETUP = 0, 1, 2, 3, 4
+STAGE_VOLUME_COUNT, STAGE_VOLUME_SCAN, STAGE_VOLUME_EMIT, STAGE_VOLUME_TABLES, STAGE_COUNT = 5, 6, 7, 8, 9
 CHUNK_INVALID = 8
 
 
@@ -106,6 +109,7 @@ SIGNATURES = {
     "uw_permutation_table": (None, [ctypes.c_uint32, P(ctypes.c_uint8)]),
     "uw_profile_enable": (ctypes.c_int, [c_void_p, ctypes.c_int]),
     "uw_profile_read": (ctypes.c_int, [c_void_p, P(ctypes.c_double), P(ctypes.c_uint64)]),
+    "uw_profile_read_stages": (ctypes.c_int, [c_void_p, ctypes.c_int32, P(ctypes.c_double), P(ctypes.c_uint64)]),
     "uw_max_verts": (ctypes.c_uint64, [ctypes.c_int32]),
     "uw_max_inds": (ctypes.c_uint64, [ctypes.c_int32]),
     "uw_sample_density": (ctypes.c_int, [c_void_p, c_void_p, ctypes.c_int32, ctypes.c_int32, c_void_p, c_void_p,

@@ -968,25 +968,11 @@ struct BatchLaunch {
     size_t dscr_bytes;     // density scratch of the whole grid
 };
 
-int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
+// Launch shape of a batched launch (kernel, LDS, grid, ring, scratch sizes); allocates nothing.
+int shape_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
     const uw_batch_tuning& tune = ctx->batch_tuning;
     const int32_t octaves = ctx->params.octaves;
-    if (size_t(nchunks) > ctx->batch_order_cap) {
-        (void)hipFree(ctx->d_batch_order);
-        ctx->d_batch_order = nullptr;
-        ctx->batch_order_cap = 0;
-        UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, order_bufs(nchunks).bytes));
-        ctx->batch_order_cap = size_t(nchunks);
-    }
-    if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, kQueueBytes));
     bl->ztab_bytes = size_t(nchunks) * uw_ztab_bytes(octaves, tune.max_res + 1);
-    if (bl->ztab_bytes > ctx->batch_ztab_bytes) {
-        (void)hipFree(ctx->d_batch_ztab);
-        ctx->d_batch_ztab = nullptr;
-        ctx->batch_ztab_bytes = 0;
-        UW_HIP(ctx, hipMalloc(&ctx->d_batch_ztab, bl->ztab_bytes));
-        ctx->batch_ztab_bytes = bl->ztab_bytes;
-    }
     bl->k = pick_kernel(tune.block, octaves);
     const int32_t list_cap = list_cap_for(tune);
     const bool one_pass = one_pass_only(tune, list_cap);
@@ -1025,6 +1011,27 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
     const uint64_t Lm = uint64_t(tune.max_res) + 1;
     bl->dscr_stride = slabs ? Lm * Lm * Lm : 0;
     bl->dscr_bytes = size_t(bl->dscr_stride) * size_t(bl->grid) * sizeof(float);
+    return UW_OK;
+}
+
+// shape_batched plus the context-owned scratch of the eager path (grown as needed).
+int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
+    if (int rc = shape_batched(ctx, nchunks, bl)) return rc;
+    if (size_t(nchunks) > ctx->batch_order_cap) {
+        (void)hipFree(ctx->d_batch_order);
+        ctx->d_batch_order = nullptr;
+        ctx->batch_order_cap = 0;
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_order, order_bufs(nchunks).bytes));
+        ctx->batch_order_cap = size_t(nchunks);
+    }
+    if (!ctx->d_batch_head) UW_HIP(ctx, hipMalloc(&ctx->d_batch_head, kQueueBytes));
+    if (bl->ztab_bytes > ctx->batch_ztab_bytes) {
+        (void)hipFree(ctx->d_batch_ztab);
+        ctx->d_batch_ztab = nullptr;
+        ctx->batch_ztab_bytes = 0;
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_ztab, bl->ztab_bytes));
+        ctx->batch_ztab_bytes = bl->ztab_bytes;
+    }
     if (bl->dscr_bytes > ctx->batch_dscr_bytes) {
         (void)hipFree(ctx->d_batch_dscr);
         ctx->d_batch_dscr = nullptr;
@@ -1077,10 +1084,12 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     const OrderBufs ob = order_bufs(nchunks);
     uint32_t* keys = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(task_buf) + ob.keys_off);
     uint32_t* hist = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(task_buf) + ob.hist_off);
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_BATCH_SETUP)) return rc;
     hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks) + uint32_t(nest)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords,
                        res, nchunks, keys, hist, nest, a.zstride, ztab);
     hipLaunchKernelGGL(k_place, dim3(uint32_t(nest)), dim3(kPrepThreads), 0, st, coords, res, nchunks, keys, hist, nest,
                        tasks, head, a.totals, uint32_t(bl.grid));
+    if (int rc = uw_prof_end(ctx, st)) return rc;
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;

@@ -27,7 +27,7 @@ int uw_refuse_capture(uw_ctx* ctx, hipStream_t st) {
     return UW_OK;
 }
 
-int uw_prof_start(uw_ctx* ctx, hipStream_t st) {
+int uw_prof_start(uw_ctx* ctx, hipStream_t st, int stage) {
     if (!ctx->profiling) return UW_OK;
     if (ctx->prof_used + 2 > ctx->prof_events.size()) {
         for (int i = 0; i < 2; ++i) {
@@ -36,6 +36,8 @@ int uw_prof_start(uw_ctx* ctx, hipStream_t st) {
             ctx->prof_events.push_back(e);
         }
     }
+    if (ctx->prof_stage.size() < ctx->prof_events.size() / 2) ctx->prof_stage.resize(ctx->prof_events.size() / 2);
+    ctx->prof_stage[ctx->prof_used / 2] = stage;
     UW_HIP(ctx, hipEventRecord(ctx->prof_events[ctx->prof_used], st));
     return UW_OK;
 }
@@ -267,19 +269,27 @@ int uw_profile_enable(uw_ctx* ctx, int on) {
     return UW_OK;
 }
 
-int uw_profile_read(uw_ctx* ctx, double* total_ms, uint64_t* launches) {
-    if (!ctx || !total_ms || !launches) return UW_ERR_INVALID_ARG;
-    double sum = 0.0;
+int uw_profile_read_stages(uw_ctx* ctx, int32_t nstages, double* total_ms, uint64_t* launches) {
+    if (!ctx || nstages < 1 || !total_ms || !launches) return UW_ERR_INVALID_ARG;
+    for (int32_t k = 0; k < nstages; ++k) {
+        total_ms[k] = 0.0;
+        launches[k] = 0;
+    }
     for (size_t i = 0; i + 1 < ctx->prof_used; i += 2) {
         UW_HIP(ctx, hipEventSynchronize(ctx->prof_events[i + 1]));
+        const int k = ctx->prof_stage[i / 2];
+        if (k < 0 || k >= nstages) continue;
         float ms = 0.0f;
         UW_HIP(ctx, hipEventElapsedTime(&ms, ctx->prof_events[i], ctx->prof_events[i + 1]));
-        sum += ms;
+        total_ms[k] += ms;
+        launches[k] += 1;
     }
-    *total_ms = sum;
-    *launches = ctx->prof_used / 2;
     ctx->prof_used = 0;
     return UW_OK;
+}
+
+int uw_profile_read(uw_ctx* ctx, double* total_ms, uint64_t* launches) {
+    return uw_profile_read_stages(ctx, 1, total_ms, launches);
 }
 
 uint64_t uw_max_verts(int32_t n) { return n < 1 ? 0 : 5ull * uint64_t(n) * uint64_t(n + 1) * uint64_t(n + 1); }

@@ -74,6 +74,7 @@ struct uw_ctx {
     // kernel timing (uw_profile_*)
     bool profiling = false;
     std::vector<hipEvent_t> prof_events;  // pairs (start, end), reused
+    std::vector<int> prof_stage;          // stage (enum uw_profile_stage) of each recorded pair
     size_t prof_used = 0;                 // events recorded since the last read
     // single-chunk host facade buffers
     int32_t* d_coord1 = nullptr;
@@ -92,7 +93,8 @@ int uw_fail(uw_ctx* ctx, int code, const std::string& msg);
 // context-owned scratch (reallocated by later calls that need more) must not be captured.
 int uw_refuse_capture(uw_ctx* ctx, hipStream_t st);
 // Record the start / end event of a timed kernel on `st` (no-ops unless profiling is enabled).
-int uw_prof_start(uw_ctx* ctx, hipStream_t st);
+// stage: enum uw_profile_stage (0 = the path's dominant kernel).
+int uw_prof_start(uw_ctx* ctx, hipStream_t st, int stage = 0);
 int uw_prof_end(uw_ctx* ctx, hipStream_t st);
 int uw_hip_fail(uw_ctx* ctx, hipError_t e, const char* what);
 

@@ -438,14 +438,20 @@ int uw_build_chunks_plain(uw_ctx* ctx, const int32_t* coords, int32_t nchunks, i
     const uw_field f = uw_make_field(ctx->params);
     if (int rc = launch_noise(ctx, f, coords, nchunks, n, s.dens, s.below, st)) return rc;
     uint8_t* cs = cases ? cases : s.cases;
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_PLAIN_CLASSIFY)) return rc;
     hipLaunchKernelGGL(k_classify, dim3(uint32_t(nchunks)), dim3(kThreads), 0, st, f.iso, ctx->d_tables, n, s.below,
                        s.dens, cs, s.voff, s.act, s.ioff, out->info);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_PLAIN_OFFSETS)) return rc;
     hipLaunchKernelGGL(k_offsets, dim3(1), dim3(1024), 0, st, out->info, nchunks, out->vert_cap, out->ind_cap,
                        out->index_format == UW_INDEX_U16 ? 1u : 0u, out->totals);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
     const float step = float(f.extent) / float(n);
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_PLAIN_EMIT)) return rc;
     hipLaunchKernelGGL(k_emit, dim3(uint32_t(nchunks)), dim3(kThreads), 0, st, f.iso, f.extent, ctx->d_tables, coords,
                        n, step, s.dens, cs, s.voff, s.act, s.ioff, out->info, out->verts, out->inds,
                        uw_make_emit_opts(out));
+    if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
 }

@@ -704,10 +704,11 @@ VolDensityKernel pick_vol_density(int octaves) {
 int vol_launch_tables(uw_ctx* ctx, VolArgs& a, const VolLayout& lay, hipStream_t st) {
     unsigned char* base = ctx->d_vol;
     const int64_t work = int64_t(3) * a.f.octaves * a.L + a.L + 1;
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_VOLUME_TABLES)) return rc;
     hipLaunchKernelGGL(k_vol_tables, dim3(uint32_t((work + kVolBlock - 1) / kVolBlock)), dim3(kVolBlock), 0, st, a,
                        reinterpret_cast<uw_axis*>(base + lay.axis_off), reinterpret_cast<float*>(base + lay.df_off),
                        reinterpret_cast<uint32_t*>(base + lay.zrec_off));
-    return UW_OK;
+    return uw_prof_end(ctx, st);
 }
 
 bool vol_args_ok(int32_t n, const uw_mesh_out* out) {
@@ -747,11 +748,17 @@ int uw_build_volume(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, 
     const uint64_t dblocks = (lines + kVolBlock - 1) / kVolBlock * uint64_t(a.zseg);
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(dk, dim3(uint32_t(dblocks)), dim3(kVolBlock), lds, st, a);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
     const uint32_t tiles = uint32_t(a.T) * uint32_t(a.T) * uint32_t(a.T);
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_VOLUME_COUNT)) return rc;
     hipLaunchKernelGGL(k_vol_count, dim3(tiles), dim3(kVolBlock), 0, st, a);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_VOLUME_SCAN)) return rc;
     hipLaunchKernelGGL(k_vol_scan_reduce, dim3(lay.nscan), dim3(kVolBlock), 0, st, a.rows, lay.nrows, a.bsum);
     hipLaunchKernelGGL(k_vol_scan_blocks, dim3(1), dim3(kVolBlock), 0, st, a, lay.nscan);
     hipLaunchKernelGGL(k_vol_scan_apply, dim3(lay.nscan), dim3(kVolBlock), 0, st, a.rows, lay.nrows, a.bsum);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_VOLUME_EMIT)) return rc;
     hipLaunchKernelGGL(k_vol_emit, dim3(tiles), dim3(kVolBlock), 0, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());
