@@ -256,6 +256,20 @@ int uw_set_batch_tuning(uw_ctx* ctx, const uw_batch_tuning* t);
 int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
                             const uw_mesh_out* out, void* stream);
 
+/* No counterpart in the reference: uw_build_chunks_batched for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). It calls
+ * no hipMalloc / hipFree: its scratch (queue state, task records, z tables) lives in a file-scope
+ * device array of UW_NOALLOC_SCRATCH_BYTES per device, held until the process ends, and x-slab
+ * chunks evaluate their densities twice instead of keeping them in a density scratch. Launches
+ * needing more scratch than that are refused with UW_ERR_INVALID_ARG (uw_last_error gives the size
+ * needed); about 8,000 chunks of up to 32^3 cells at 6 octaves fit. Every call on one device shares
+ * that device's array, so calls that overlap on a device (concurrent streams, concurrently replayed
+ * graphs) race: the caller serialises them. May be captured into a graph (the launches bind no
+ * context-owned buffer). Output as for uw_build_chunks_batched. */
+#define UW_NOALLOC_SCRATCH_BYTES (32u << 20)
+int uw_build_chunks_batched_noalloc(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                                    const uw_mesh_out* out, void* stream);
+
 /* ---- Differentiation (no counterpart in the reference, which has no gradients) ------------ */
 
 /* Replace the octave amplitudes (octaves values) and/or the world-space sample offset (3 values)

@@ -59,6 +59,7 @@ class UwTotals(ctypes.Structure):
 
 BATCH_MAX_RESOLUTION = 32
 VOLUME_MAX_RESOLUTION = 640
+NOALLOC_SCRATCH_BYTES = 32 << 20
 # enum uw_profile_stage
 STAGE_DOMINANT, STAGE_PLAIN_CLASSIFY, STAGE_PLAIN_OFFSETS, STAGE_PLAIN_EMIT, STAGE_BATCH_SETUP = 0, 1, 2, 3, 4
 STAGE_VOLUME_COUNT, STAGE_VOLUME_SCAN, STAGE_VOLUME_EMIT, STAGE_VOLUME_TABLES, STAGE_COUNT = 5, 6, 7, 8, 9
@@ -126,6 +127,8 @@ SIGNATURES = {
     "uw_get_batch_tuning": (ctypes.c_int, [c_void_p, P(UwBatchTuning)]),
     "uw_set_batch_tuning": (ctypes.c_int, [c_void_p, P(UwBatchTuning)]),
     "uw_build_chunks_batched": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut), c_void_p]),
+    "uw_build_chunks_batched_noalloc": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut),
+                                                       c_void_p]),
     "uw_batched_graph_create": (ctypes.c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_int32, P(UwMeshOut),
                                                P(c_void_p)]),
     "uw_graph_launch": (ctypes.c_int, [c_void_p, c_void_p]),

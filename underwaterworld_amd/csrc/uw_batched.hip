@@ -19,6 +19,8 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "uw_mesh.h"
 #include "uw_noise.h"
 #include "uw_wave.h"
@@ -1210,6 +1212,84 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     out_g->buffers = gbuf;
     *g = out_g;
     return UW_OK;
+}
+
+}  // extern "C"
+
+// ---- No-allocation form -------------------------------------------------------------------------
+// The batched build with its scratch (queue state, task records with their ordering keys and
+// histograms, z tables) in one file-scope device array instead of context-owned allocations, for
+// callers that cannot allocate (code under graph capture, tight serving loops). x-slab chunks
+// evaluate their densities again in the emission pass instead of keeping them in a density scratch.
+// Each device holds its own copy of the array until the process ends; its device address is taken
+// once per device when a context is created on it. Every call on a device shares that device's
+// array: calls that overlap on one device (concurrent streams, or graphs replayed concurrently)
+// race, and must be serialised by the caller. Each launch rewrites everything it reads (the queue
+// set-up kernels reset the counters and write every record), so no clearing is needed between calls.
+namespace {
+constexpr size_t kNoAllocBytes = size_t(32) << 20;  // UW_NOALLOC_SCRATCH_BYTES
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+unsigned char* g_scratch_ptr[kNoAllocDevices] = {};
+std::mutex g_scratch_mu;
+
+size_t noalloc_bytes(const BatchLaunch& bl, int32_t nchunks) {
+    return kQueueBytes + ((order_bufs(nchunks).bytes + 255) & ~size_t(255)) + ((bl.ztab_bytes + 255) & ~size_t(255));
+}
+}  // namespace
+
+// Device address of this device's scratch array (once per device, with the device current).
+int uw_noalloc_setup(uw_ctx* ctx) {
+    static_assert(kNoAllocBytes == UW_NOALLOC_SCRATCH_BYTES, "scratch size and header disagree");
+    if (ctx->device < 0 || ctx->device >= kNoAllocDevices) return UW_OK;  // the no-alloc form refuses such devices
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[ctx->device]) return UW_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) return uw_hip_fail(ctx, e != hipSuccess ? e : hipErrorInvalidSymbol, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[ctx->device] = static_cast<unsigned char*>(ptr);
+    return UW_OK;
+}
+
+extern "C" {
+
+int uw_build_chunks_batched_noalloc(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
+                                    const uw_mesh_out* out, void* stream) {
+    if (!ctx) return UW_ERR_INVALID_ARG;
+    if (!args_ok(nchunks, coords, res, out)) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bad arguments");
+    if (ctx->device < 0 || ctx->device >= kNoAllocDevices)
+        return uw_fail(ctx, UW_ERR_INVALID_ARG, "no-allocation scratch: device index out of range");
+    UW_HIP(ctx, hipSetDevice(ctx->device));
+    unsigned char* base;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        base = g_scratch_ptr[ctx->device];
+    }
+    if (!base) return uw_fail(ctx, UW_ERR_INVALID_ARG, "no-allocation scratch not set up on this device");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    BatchLaunch bl;
+    if (int rc = shape_batched(ctx, nchunks > 0 ? nchunks : 1, &bl)) return rc;
+    const size_t need = noalloc_bytes(bl, nchunks > 0 ? nchunks : 1);
+    if (need > kNoAllocBytes)
+        return uw_fail(ctx, UW_ERR_INVALID_ARG,
+                       "launch needs " + std::to_string(need) + " bytes of scratch; the no-allocation form has " +
+                           std::to_string(kNoAllocBytes) + " (fewer chunks, a smaller max_res or fewer octaves fit)");
+    uint32_t* head = reinterpret_cast<uint32_t*>(base);
+    if (nchunks == 0) {
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, st, head, out->totals);
+        UW_HIP(ctx, hipGetLastError());
+        return UW_OK;
+    }
+    unsigned char* tasks = base + kQueueBytes;
+    unsigned char* ztab = tasks + ((order_bufs(nchunks).bytes + 255) & ~size_t(255));
+    // timing events only outside graph capture
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (st) UW_HIP(ctx, hipStreamIsCapturing(st, &cs));
+    const bool was_profiling = ctx->profiling;
+    if (cs != hipStreamCaptureStatusNone) ctx->profiling = false;
+    const int rc = launch_batched(ctx, bl, coords, res, nchunks, out, head, tasks, ztab, nullptr, st);
+    ctx->profiling = was_profiling;
+    return rc;
 }
 
 int uw_graph_launch(uw_graph* g, void* stream) {

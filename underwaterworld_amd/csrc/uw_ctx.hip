@@ -214,7 +214,8 @@ int uw_ctx_create(const uw_params* p, int device, uw_ctx** out) {
         ctx->num_cus = prop.multiProcessorCount;
     if (hipSetDevice(device) != hipSuccess ||
         hipMalloc(&ctx->d_tables, sizeof(uw_tables)) != hipSuccess ||
-        hipMemcpy(ctx->d_tables, &ctx->host_tables, sizeof(uw_tables), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(ctx->d_tables, &ctx->host_tables, sizeof(uw_tables), hipMemcpyHostToDevice) != hipSuccess ||
+        uw_noalloc_setup(ctx) != UW_OK) {
         uw_ctx_destroy(ctx);
         return UW_ERR_HIP;
     }

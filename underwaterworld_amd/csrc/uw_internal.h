@@ -89,6 +89,9 @@ struct uw_ctx {
 };
 
 int uw_fail(uw_ctx* ctx, int code, const std::string& msg);
+// Takes the device address of the no-allocation scratch array on the context's (current) device,
+// once per device.
+int uw_noalloc_setup(uw_ctx* ctx);
 // UW_ERR_INVALID_ARG when `st` is capturing a graph: entry points whose launches bind
 // context-owned scratch (reallocated by later calls that need more) must not be captured.
 int uw_refuse_capture(uw_ctx* ctx, hipStream_t st);
