@@ -28,8 +28,10 @@ def _cases(d, n, iso):
     return c
 
 
-def tiled_ids(cases, n, rows, tile):
-    """Index buffer and vertex count of the grid by the volume kernels' procedure."""
+def tiled_ids(cases, n, rows, tile, planes=None):
+    """Index buffer and vertex count of the grid by the volume kernels' procedure (emission by
+    x-slabs of `planes` cell planes of a tile, each with its own halo plane)."""
+    planes = planes or tile
     before = before_table(rows)
     used = [sum(1 << e for e in set(r)) for r in rows]
     ntri = [len(r) // 3 for r in rows]
@@ -47,12 +49,15 @@ def tiled_ids(cases, n, rows, tile):
     base = (np.cumsum(flat, axis=0) - flat).reshape(n, n, T, 2)
     nverts, ntris = int(flat[:, 0].sum()), int(flat[:, 1].sum())
     inds = np.full(3 * ntris, -1, np.int64)
-    # k_vol_emit, one tile at a time
+    # k_vol_emit, one x-slab of a tile at a time
     for tx in range(T):
         for ty in range(T):
-            for tz in range(T):
-                X0, Y0, Z0 = tx * tile, ty * tile, tz * tile
-                X1, Y1, Z1 = min(n, X0 + tile), min(n, Y0 + tile), min(n, Z0 + tile)
+            for tz, sl in [(tz, sl) for tz in range(T) for sl in range((tile + planes - 1) // planes)]:
+                X0, Y0, Z0 = tx * tile + sl * planes, ty * tile, tz * tile
+                X1 = min(n, tx * tile + tile, X0 + planes)
+                Y1, Z1 = min(n, Y0 + tile), min(n, Z0 + tile)
+                if X0 >= X1:
+                    continue
                 xs, ys, zs = max(X0 - 1, 0), max(Y0 - 1, 0), max(Z0 - 1, 0)
                 ny, nz = Y1 - ys, Z1 - zs
                 assert ny <= tile + 1 and nz <= tile + 1 and (ny + 1) <= tile + 2 and (nz + 1) <= tile + 2
@@ -94,14 +99,15 @@ def tiled_ids(cases, n, rows, tile):
     return inds.astype(np.uint32), nverts
 
 
-@pytest.mark.parametrize("n,tile,seed,frac", [(5, 2, 0, 0.5), (9, 4, 1, 0.5), (10, 3, 2, 0.35), (7, 7, 3, 0.6),
-                                              (8, 4, 4, 0.5), (11, 4, 5, 0.5), (1, 4, 6, 0.5)])
-def test_tiled_numbering_equals_first_seen(n, tile, seed, frac):
+@pytest.mark.parametrize("n,tile,planes,seed,frac", [(5, 2, 1, 0, 0.5), (9, 4, 2, 1, 0.5), (10, 3, 3, 2, 0.35),
+                                                     (7, 7, 7, 3, 0.6), (8, 4, 1, 4, 0.5), (11, 4, 4, 5, 0.5),
+                                                     (1, 4, 1, 6, 0.5), (13, 8, 4, 7, 0.5)])
+def test_tiled_numbering_equals_first_seen(n, tile, planes, seed, frac):
     rows = packed_rows()
     p = PR.preset("chunks12")
     d = (np.random.default_rng(seed).random((n + 1) ** 3) - frac).astype(np.float32)
     m = O.mesh_ref(p, (0, 0, 0), n, d)
-    ids, nv = tiled_ids(_cases(d, n, 0.0), n, rows, tile)
+    ids, nv = tiled_ids(_cases(d, n, 0.0), n, rows, tile, planes)
     assert nv == len(m["verts"])
     assert np.array_equal(ids, m["inds"])
 

@@ -195,27 +195,33 @@ __device__ __forceinline__ void tile_corners(const float* p0, const float* p1, i
     r[7] = p0[o + kPitch + 1];
 }
 
-__device__ __forceinline__ void tile_of(const VolArgs& a, int32_t* X0, int32_t* Y0, int32_t* Z0, int32_t* X1,
-                                        int32_t* Y1, int32_t* Z1, int32_t* tz) {
+// Workgroup b takes x-slab (b % slabs) of tile b / slabs: cell planes [X0, X1) of `planes` each.
+__device__ __forceinline__ void tile_of(const VolArgs& a, int32_t planes, int32_t* X0, int32_t* Y0, int32_t* Z0,
+                                        int32_t* X1, int32_t* Y1, int32_t* Z1, int32_t* tz) {
     const int32_t T = a.T;
-    const int32_t b = int32_t(blockIdx.x);
+    const int32_t slabs = kTile / planes;
+    const int32_t b = int32_t(blockIdx.x) / slabs, sl = int32_t(blockIdx.x) - b * slabs;
     const int32_t tx = b / (T * T), ty = (b / T) % T;
     *tz = b % T;
-    *X0 = tx * kTile;
+    *X0 = tx * kTile + sl * planes;
     *Y0 = ty * kTile;
     *Z0 = *tz * kTile;
-    *X1 = min(a.n, *X0 + kTile);
+    *X1 = min(min(a.n, tx * kTile + kTile), *X0 + planes);
     *Y1 = min(a.n, *Y0 + kTile);
     *Z1 = min(a.n, *Z0 + kTile);
 }
 
-// Per-row-segment counts of one tile.
+constexpr int32_t kCountPlanes = 8;  // cell planes per count workgroup (a quarter of a tile)
+constexpr int32_t kEmitPlanes = 4;   // cell planes emitted per emit workgroup (plus one halo plane)
+
+// Per-row-segment counts of one x-slab of a tile.
 __global__ void __launch_bounds__(kVolBlock) k_vol_count(VolArgs a) {
     __shared__ float SP[2][kTileS * kPitch];
     __shared__ uint32_t LUT[256];
     __shared__ uint32_t s_act;
     int32_t X0, Y0, Z0, X1, Y1, Z1, tz;
-    tile_of(a, &X0, &Y0, &Z0, &X1, &Y1, &Z1, &tz);
+    tile_of(a, kCountPlanes, &X0, &Y0, &Z0, &X1, &Y1, &Z1, &tz);
+    if (X0 >= X1) return;  // a slab past the grid's last plane (whole workgroup)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < 256; i += kVolBlock) LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(a.tab->tri[i] & 15u) << 16);
     if (tid == 0) s_act = 0u;
@@ -337,26 +343,32 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_scan_apply(unsigned long long
 }
 
 // ---------------------------------------------------------------------------------------------
-// Emission of one tile. Cell planes x = xs .. X1 - 1 (xs = X0 - 1 when a plane below the tile
-// exists: its cells are classified for owner lookups only); in each plane the rows y = ys .. Y1 - 1
-// and the cells z = zs .. Z1 - 1, likewise with halo row / column.
+// Emission of one x-slab of a tile (cell planes [X0, X1)). Cell planes x = xs .. X1 - 1 (xs = X0 - 1
+// when a plane below the slab exists: its cells are classified for owner lookups only); in each
+// plane the rows y = ys .. Y1 - 1 and the cells z = zs .. Z1 - 1, likewise with halo row / column.
+// Per emitted plane the active cells are compacted into a list (LDS atomics per row, any order:
+// every id and index position comes from the row bases), then one lane per listed cell writes its
+// vertices and one lane per triangle its three index entries, so no lane idles on an empty cell or
+// waits for the cell with the most triangles.
+constexpr int kT2C = 2048;  // triangles of a plane taken one per lane (more: one lane per cell)
+
 __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
     __shared__ float SP[2][kTileS * kPitch];
     __shared__ uint32_t CB[2][(kTile + 1) * (kTile + 1)];  // vertex id of the cell's first owned record
     __shared__ uint8_t CC[2][(kTile + 1) * (kTile + 1)];   // cube case
-    __shared__ uint32_t CI[kTile * kTile];                  // first index of the cell (emitted cells)
-    __shared__ uint64_t TRI[256];
+    __shared__ uint32_t LIST[kTile * kTile];  // active cells of the plane: (y - Y0) 32 + z - Z0 | triangle prefix << 10
+    __shared__ uint32_t LIB[kTile * kTile];   // index position of each listed cell's first index entry
+    __shared__ uint16_t T2C[kT2C];            // plane-local triangle -> list slot
     __shared__ uint32_t LUT[256];
     __shared__ uint16_t BEF[256 * 12];
+    __shared__ uint32_t s_cnt;  // listed cells | triangles << 16
     const uw_totals tot = *a.totals;
     if (tot.overflow || tot.n_inds == 0) return;
     int32_t X0, Y0, Z0, X1, Y1, Z1, tz;
-    tile_of(a, &X0, &Y0, &Z0, &X1, &Y1, &Z1, &tz);
+    tile_of(a, kEmitPlanes, &X0, &Y0, &Z0, &X1, &Y1, &Z1, &tz);
+    if (X0 >= X1) return;  // a slab past the grid's last plane (whole workgroup)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < 256; i += kVolBlock) {
-        TRI[i] = a.tab->tri[i];
-        LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(a.tab->tri[i] & 15u) << 16);
-    }
+    for (int i = tid; i < 256; i += kVolBlock) LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(a.tab->tri[i] & 15u) << 16);
     for (int i = tid; i < 256 * 12; i += kVolBlock) BEF[i] = (&a.tab->before[0][0])[i];
     const int32_t xs = X0 > 0 ? X0 - 1 : X0, ys = Y0 > 0 ? Y0 - 1 : Y0, zs = Z0 > 0 ? Z0 - 1 : Z0;
     const int32_t ny = Y1 - ys, nz = Z1 - zs;  // cells per plane row / column, halo included
@@ -367,16 +379,20 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
     const uint64_t vlim = tot.n_verts, ilim = tot.n_inds;
     const int32_t L = a.L, n = a.n;
     const uw_emit_opts& eo = a.eo;
+    const bool coll = eo.tri_normals || eo.tri_cells;
+    const uint64_t* __restrict__ tri_rows = a.tab->tri;
     load_plane(a, xs, ys, sy, zs, sz, SP[xs & 1]);
     for (int32_t x = xs; x < X1; ++x) {
         load_plane(a, x + 1, ys, sy, zs, sz, SP[(x + 1) & 1]);
+        if (tid == 0) s_cnt = 0u;
         __syncthreads();
         const float* p0 = SP[x & 1];
         const float* p1 = SP[(x + 1) & 1];
         uint32_t* cb = CB[x & 1];
         uint8_t* cc = CC[x & 1];
         // classification, vertex and index bases of the plane's rows (cells z >= Z0 of a row lie
-        // in the row segment (x, y, tz); the halo cell z = Z0 - 1 is the last of segment tz - 1)
+        // in the row segment (x, y, tz); the halo cell z = Z0 - 1 is the last of segment tz - 1),
+        // and the list of the emitted cells
         for (int32_t q = wave; 2 * q < ny; q += kVolBlock / 64) {
             const int32_t r = 2 * q + (lane >> 5);
             const int32_t y = ys + r, z = Z0 + (lane & 31);
@@ -387,16 +403,28 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
                 tile_corners(p0, p1, y, z, ys, zs, rr);
                 cs = uw_case_of(rr, iso);
                 const uint32_t w = LUT[cs];
-                v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | ((w >> 16) << 10);
+                const uint32_t nt = w >> 16;
+                // records (< 2^10), triangles (< 2^10) and active cells (< 2^6) of a row of 32
+                v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | (nt << 10) | ((nt ? 1u : 0u) << 20);
             }
             const uint32_t inc = uw_half_incl_scan(v);
+            const bool emitted = x >= X0 && y >= Y0 && y < Y1;
+            uint32_t old = 0;
+            if ((lane & 31) == 31 && emitted && (inc >> 20))
+                old = atomicAdd(&s_cnt, (inc >> 20) | (((inc >> 10) & 0x3FFu) << 16));
+            const uint32_t lbase = (lane & 32) ? uint32_t(__builtin_amdgcn_readlane(int(old), 63))
+                                               : uint32_t(__builtin_amdgcn_readlane(int(old), 31));
             if (in) {
                 const unsigned long long rb = a.rows[(size_t(x) * n + y) * a.T + tz];
                 const uint32_t ex = inc - v;
                 const int o = r * nz + (z - zs);
                 cb[o] = uint32_t(rb) + (ex & 0x3FFu);
                 cc[o] = uint8_t(cs);
-                if (x >= X0 && y >= Y0) CI[(y - Y0) * kTile + (z - Z0)] = 3u * (uint32_t(rb >> 32) + (ex >> 10));
+                if (emitted && (v >> 20)) {
+                    const uint32_t slot = (lbase & 0xFFFFu) + (ex >> 20);
+                    LIST[slot] = uint32_t((y - Y0) * kTile + (z - Z0)) | (((lbase >> 16) + ((ex >> 10) & 0x3FFu)) << 10);
+                    LIB[slot] = 3u * (uint32_t(rb >> 32) + ((ex >> 10) & 0x3FFu));
+                }
             }
         }
         if (zs < Z0) {
@@ -413,12 +441,47 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
         }
         __syncthreads();
         if (x >= X0) {
-            for (int32_t e = tid; e < kTile * kTile; e += kVolBlock) {
-                const int32_t y = Y0 + (e >> 5), z = Z0 + (e & 31);
-                if (y >= Y1 || z >= Z1) continue;
+            const uint32_t cnt = s_cnt;
+            const uint32_t na = cnt & 0xFFFFu, ntp = cnt >> 16;
+            const bool flat = ntp <= uint32_t(kT2C);  // block-uniform
+            // record id of edge ed as seen from cell (y, z) of plane x: its own rank, or its owner's
+            auto edge_id = [&](int32_t y, int32_t z, uint32_t cs, uint32_t owned, uint32_t myv, uint32_t ed) -> uint32_t {
+                if (owned & (1u << ed)) return myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
+                int32_t qx, qy, qz;
+                uint32_t qe;
+                uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
+                const int qo = (qy - ys) * nz + (qz - zs);
+                const uint32_t qc = CC[qx & 1][qo];
+                return CB[qx & 1][qo] + __popc(uint32_t(BEF[qc * 12 + qe]) & uw_owned_mask(qx, qy, qz));
+            };
+            // index entries (and collision records) of triangle j of a listed cell
+            auto emit_tri = [&](int32_t y, int32_t z, uint32_t cs, uint32_t owned, uint32_t myv, uint64_t row, uint32_t j,
+                                uint64_t gi, const float* r) {
+                float q[3][3];
+#pragma unroll
+                for (uint32_t k = 0; k < 3; ++k) {
+                    const uint32_t ed = uint32_t(row >> (4 + 4 * (3 * j + k))) & 15u;
+                    const uint32_t id = edge_id(y, z, cs, owned, myv, ed);
+                    if (gi + k < ilim) uw_store_index(a.inds, eo.ind16, gi + k, id);
+                    if (coll) uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, q[k]);
+                }
+                if (coll && gi + 2 < ilim) {
+                    const uint64_t t = gi / 3;
+                    if (eo.tri_normals) {
+                        float nm[3];
+                        uw_tri_normal(q[0], q[1], q[2], nm);
+                        eo.tri_normals[3 * t] = nm[0];
+                        eo.tri_normals[3 * t + 1] = nm[1];
+                        eo.tri_normals[3 * t + 2] = nm[2];
+                    }
+                    if (eo.tri_cells) eo.tri_cells[t] = uint32_t((uint64_t(x) * n + y) * n + z);
+                }
+            };
+            for (uint32_t k = tid; k < na; k += kVolBlock) {
+                const uint32_t ent = LIST[k];
+                const int32_t y = Y0 + int32_t((ent & 0x3FFu) >> 5), z = Z0 + int32_t(ent & 31u);
                 const int o = (y - ys) * nz + (z - zs);
                 const uint32_t cs = cc[o];
-                if (cs == 0u || cs == 255u) continue;
                 float r[8];
                 tile_corners(p0, p1, y, z, ys, zs, r);
                 const uint32_t owned = uw_owned_mask(x, y, z);
@@ -447,52 +510,31 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
                         a.tape[id] = tv;
                     }
                 }
-                const uint64_t row = TRI[cs];
-                const int nidx = 3 * int(row & 15u);
-                const uint64_t ib = CI[(y - Y0) * kTile + (z - Z0)];
-                const bool coll = eo.tri_normals || eo.tri_cells;
-                float q0[3] = {0.f, 0.f, 0.f}, q1[3] = {0.f, 0.f, 0.f};
-                for (int j = 0; j < nidx; ++j) {
-                    const uint32_t ed = uint32_t(row >> (4 + 4 * j)) & 15u;
-                    uint32_t id;
-                    if (owned & (1u << ed)) {
-                        id = myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
-                    } else {
-                        int32_t qx, qy, qz;
-                        uint32_t qe;
-                        uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
-                        const int qo = (qy - ys) * nz + (qz - zs);
-                        const uint32_t qc = CC[qx & 1][qo];
-                        id = CB[qx & 1][qo] + __popc(uint32_t(BEF[qc * 12 + qe]) & uw_owned_mask(qx, qy, qz));
-                    }
-                    if (ib + j >= ilim) continue;
-                    uw_store_index(a.inds, eo.ind16, ib + j, id);
-                    if (coll) {
-                        float p[3];
-                        uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
-                        const int k = j % 3;
-                        if (k == 0) {
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) q0[d] = p[d];
-                        } else if (k == 1) {
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) q1[d] = p[d];
-                        } else {
-                            const uint64_t t = (ib + j) / 3;
-                            if (eo.tri_normals) {
-                                float nm[3];
-                                uw_tri_normal(q0, q1, p, nm);
-                                eo.tri_normals[3 * t] = nm[0];
-                                eo.tri_normals[3 * t + 1] = nm[1];
-                                eo.tri_normals[3 * t + 2] = nm[2];
-                            }
-                            if (eo.tri_cells) eo.tri_cells[t] = uint32_t((uint64_t(x) * n + y) * n + z);
-                        }
-                    }
+                const uint32_t ntri = LUT[cs] >> 16;
+                if (flat) {
+                    const uint32_t tp = ent >> 10;
+                    for (uint32_t i = 0; i < ntri; ++i) T2C[tp + i] = uint16_t(k);
+                } else {
+                    const uint64_t row = tri_rows[cs];
+                    for (uint32_t j = 0; j < ntri; ++j) emit_tri(y, z, cs, owned, myv, row, j, uint64_t(LIB[k]) + 3 * j, r);
+                }
+            }
+            if (flat) {
+                __syncthreads();  // the triangle map is complete
+                for (uint32_t t = tid; t < ntp; t += kVolBlock) {
+                    const uint32_t k = T2C[t];
+                    const uint32_t ent = LIST[k];
+                    const int32_t y = Y0 + int32_t((ent & 0x3FFu) >> 5), z = Z0 + int32_t(ent & 31u);
+                    const int o = (y - ys) * nz + (z - zs);
+                    const uint32_t cs = cc[o];
+                    const uint32_t j = t - (ent >> 10);
+                    float r[8];
+                    if (coll) tile_corners(p0, p1, y, z, ys, zs, r);
+                    emit_tri(y, z, cs, uw_owned_mask(x, y, z), cb[o], tri_rows[cs], j, uint64_t(LIB[k]) + 3 * j, r);
                 }
             }
         }
-        __syncthreads();  // plane x - 1's cells and sample plane x are read no more
+        __syncthreads();  // plane x - 1's cells, sample plane x and the plane's lists are read no more
     }
 }
 
@@ -751,7 +793,7 @@ int uw_build_volume(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, 
     if (int rc = uw_prof_end(ctx, st)) return rc;
     const uint32_t tiles = uint32_t(a.T) * uint32_t(a.T) * uint32_t(a.T);
     if (int rc = uw_prof_start(ctx, st, UW_STAGE_VOLUME_COUNT)) return rc;
-    hipLaunchKernelGGL(k_vol_count, dim3(tiles), dim3(kVolBlock), 0, st, a);
+    hipLaunchKernelGGL(k_vol_count, dim3(tiles * uint32_t(kTile / kCountPlanes)), dim3(kVolBlock), 0, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
     if (int rc = uw_prof_start(ctx, st, UW_STAGE_VOLUME_SCAN)) return rc;
     hipLaunchKernelGGL(k_vol_scan_reduce, dim3(lay.nscan), dim3(kVolBlock), 0, st, a.rows, lay.nrows, a.bsum);
@@ -759,7 +801,7 @@ int uw_build_volume(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, 
     hipLaunchKernelGGL(k_vol_scan_apply, dim3(lay.nscan), dim3(kVolBlock), 0, st, a.rows, lay.nrows, a.bsum);
     if (int rc = uw_prof_end(ctx, st)) return rc;
     if (int rc = uw_prof_start(ctx, st, UW_STAGE_VOLUME_EMIT)) return rc;
-    hipLaunchKernelGGL(k_vol_emit, dim3(tiles), dim3(kVolBlock), 0, st, a);
+    hipLaunchKernelGGL(k_vol_emit, dim3(tiles * uint32_t(kTile / kEmitPlanes)), dim3(kVolBlock), 0, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
