@@ -159,7 +159,9 @@ enum uw_profile_stage {
     UW_STAGE_VOLUME_SCAN = 6,    /* k_vol_scan_* */
     UW_STAGE_VOLUME_EMIT = 7,    /* k_vol_emit */
     UW_STAGE_VOLUME_TABLES = 8,  /* k_vol_tables */
-    UW_STAGE_COUNT = 9
+    UW_STAGE_BULK_FRONT = 9,     /* k_bulk_density + k_bulk_count + k_bulk_scan (batched bulk path) */
+    UW_STAGE_BULK_EMIT = 10,     /* k_bulk_emit */
+    UW_STAGE_COUNT = 11
 };
 int uw_profile_enable(uw_ctx* ctx, int on);
 int uw_profile_read(uw_ctx* ctx, double* total_ms, uint64_t* launches);
@@ -233,12 +235,15 @@ int uw_tris_around(const uint32_t* tri_cells, uint32_t ntris, int32_t n, float p
  * as fit on the device); max_res: largest resolution a launch accepts (sizes the LDS; larger
  * chunks are flagged UW_CHUNK_INVALID); slab[n]: cell planes per x-slab for chunks of n cells per
  * axis (slab[n] = n builds the chunk in one slab with one density evaluation; smaller slabs bound
- * LDS use and evaluate the densities twice). */
+ * LDS use and evaluate the densities twice); bulk_min: chunks of at least bulk_min cells per axis
+ * (up to max_res) leave the persistent queue for the bulk path, whose launches spread each chunk's
+ * densities and planes over the device (0 or a value above max_res: no bulk path). */
 typedef struct uw_batch_tuning {
     int32_t block;
     int32_t grid;
     int32_t max_res;
     int8_t slab[UW_BATCH_MAX_RESOLUTION + 1];
+    int32_t bulk_min;
 } uw_batch_tuning;
 
 int uw_get_batch_tuning(const uw_ctx* ctx, uw_batch_tuning* t);

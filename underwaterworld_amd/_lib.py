@@ -62,13 +62,14 @@ VOLUME_MAX_RESOLUTION = 640
 NOALLOC_SCRATCH_BYTES = 32 << 20
 # enum uw_profile_stage
 STAGE_DOMINANT, STAGE_PLAIN_CLASSIFY, STAGE_PLAIN_OFFSETS, STAGE_PLAIN_EMIT, STAGE_BATCH_SETUP = 0, 1, 2, 3, 4
-STAGE_VOLUME_COUNT, STAGE_VOLUME_SCAN, STAGE_VOLUME_EMIT, STAGE_VOLUME_TABLES, STAGE_COUNT = 5, 6, 7, 8, 9
+STAGE_VOLUME_COUNT, STAGE_VOLUME_SCAN, STAGE_VOLUME_EMIT, STAGE_VOLUME_TABLES = 5, 6, 7, 8
+STAGE_BULK_FRONT, STAGE_BULK_EMIT, STAGE_COUNT = 9, 10, 11
 CHUNK_INVALID = 8
 
 
 class UwBatchTuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("grid", ctypes.c_int32), ("max_res", ctypes.c_int32),
-                ("slab", ctypes.c_int8 * (BATCH_MAX_RESOLUTION + 1))]
+                ("slab", ctypes.c_int8 * (BATCH_MAX_RESOLUTION + 1)), ("bulk_min", ctypes.c_int32)]
 
 
 class UwMeshOut(ctypes.Structure):

@@ -9,6 +9,8 @@ with 6 octaves, the 512^3 volume as 4096 bricks of 32^3 with 8 octaves) it times
             is built in one slab (one density evaluation); larger chunks use the largest x-slab
             that fits (densities evaluated twice)
     grid    persistent workgroups: all that fit (0)
+    bulk_min  chunks of at least this many cells per axis go to the bulk path (0: off); swept last
+            over the workload's resolutions
 with max_res set to the workload's largest resolution (it sizes the launch's LDS). The budget ties
 every resolution's x-slab size to one LDS figure; the fastest shape is then refined one x-slab
 resolution of the workload at a time (coordinate descent over that resolution's planes per slab,
@@ -119,6 +121,24 @@ def refine_slabs(perlin, coords, res, best_ms, best, steps, stream, ref, log=pri
     return best_ms, best, results
 
 
+def refine_bulk(perlin, coords, res, best_ms, best, steps, stream, ref, log=print):
+    """The bulk path's threshold: off (0), or each distinct resolution of the workload above the
+    one-pass sizes (chunks of at least bulk_min cells per axis leave the persistent queue)."""
+    results = []
+    cands = [0] + sorted({int(n) for n in res if 13 <= int(n) <= int(best["max_res"])})
+    for b in cands:
+        if b == int(best.get("bulk_min", -1)):
+            continue
+        t = dict(best, slab=list(best["slab"]), bulk_min=b)
+        ms = _time_shape(perlin, coords, res, t, steps, stream, ref, log, f"bulk_min={b}")
+        if ms is None:
+            continue
+        results.append((ms, t))
+        if ms < best_ms:
+            best_ms, best = ms, t
+    return best_ms, best, results
+
+
 def tune_workload(perlin, coords, res, steps=10, quick=False, log=print, start=None):
     import torch
 
@@ -147,6 +167,8 @@ def tune_workload(perlin, coords, res, steps=10, quick=False, log=print, start=N
         results.append((best_ms, best))
     best_ms, best, refined = refine_slabs(perlin, coords, res, best_ms, best, steps, stream, ref, log)
     results += refined
+    best_ms, best, bulk = refine_bulk(perlin, coords, res, best_ms, best, steps, stream, ref, log)
+    results += bulk
     set_batch_tuning(perlin, base)
     results.sort(key=lambda r: r[0])
     return results
@@ -179,7 +201,8 @@ def main():
         best_ms, best = r[0]
         report["by_octaves"][str(p["octaves"])] = best
         report["timings"][name] = [{"ms": round(ms, 5), "block": t["block"], "grid": t["grid"],
-                                    "slab12": t["slab"][12], "slab24": t["slab"][24], "slab32": t["slab"][32]}
+                                    "slab12": t["slab"][12], "slab24": t["slab"][24], "slab32": t["slab"][32],
+                                    "bulk_min": t.get("bulk_min", 0)}
                                    for ms, t in r]
         print(f"best {name}: {best_ms:.4f} ms block={best['block']} slab12={best['slab'][12]} "
               f"slab24={best['slab'][24]} slab32={best['slab'][32]}", flush=True)

@@ -23,6 +23,7 @@
 
 #include "uw_mesh.h"
 #include "uw_noise.h"
+#include "uw_queue.h"
 #include "uw_wave.h"
 
 namespace {
@@ -35,20 +36,7 @@ constexpr int kRingMax = 4096, kRingMin = 2048;
 constexpr int kOnePassMax = 2048; // most cells of a chunk built in one pass (the compacted list's cap)
 constexpr int kMinList = 512;     // smallest compacted active-cell list
 constexpr int kDensBudget = 20 * 1024;  // default LDS bytes for the density slab ring
-// Queue state (one allocation, 512 bytes): the task head (u32) at byte 0, the vertex range counter
-// (u64; in packed launches vertices | indices << 32) at byte 128, the index range counter (u64,
-// unpacked launches) at byte 256 and the exit counter (u64: workgroups done | non-blank chunks << 32)
-// at byte 384, each in its own cache line.
-constexpr size_t kQueueBytes = 512;
-constexpr int kResvIndsOffset = 16;  // u64 words from the vertex counter to the index counter
-constexpr int kExitOffset = 32;      // u64 words from the vertex counter to the exit counter
-
-// One queued chunk, in queue order (32 bytes, two 16-byte loads): chunk index c (-1: queue
-// empty), resolution n and chunk coordinates. Written by k_place, read once per task.
-struct __align__(16) uw_task {
-    int32_t c, n, cx, cy;
-    int32_t cz, pad[3];
-};
+constexpr int kDefaultBulkMin = 17;     // default bulk_min: the first resolution built in x-slabs
 
 struct BatchArgs {
     uw_field f;
@@ -130,9 +118,14 @@ __device__ __forceinline__ int order_bucket(const uw_field& f, const uint8_t* P,
 //           the other workgroups write the chunks' z tables, one thread per record.
 //  k_place: workgroup j places the chunks of estimate workgroup j: bucket offsets from all
 //           histograms (total over all workgroups, plus the part of workgroups before j), then
-//           slot = offset + rank. Workgroup 0 also resets the totals, the range counters and the
-//           queue head (to first_task: the persistent kernel's workgroup g starts with task g
-//           without claiming it).
+//           slot = offset + rank. Workgroup 0 also resets the totals and the range counters, and
+//           publishes the number of queue entries the bulk path takes (the chunks of buckets below
+//           bulk_bucket: resolution bulk_min and up, which come first; 0: no bulk path) and the
+//           queue head (that number plus first_task: the persistent kernel's workgroup g starts
+//           with the task after the bulk entries and g others, without claiming it). Thread c clears
+//           slot c of the bulk path's per-entry counters (early-blank bits, active cells; indexed
+//           by queue entry, so clearing every slot below nchunks covers the bulk entries; null
+//           without the bulk path).
 // The histograms are written whole (zeros included) by every launch, so no state carries over.
 // Per-chunk key: bucket | rank << 16 (rank < kPrepThreads).
 constexpr int kPrepThreads = 256;
@@ -209,14 +202,14 @@ __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restric
                                                         const uint32_t* __restrict__ keys,
                                                         const uint32_t* __restrict__ hist, int32_t nest,
                                                         uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
-                                                        uint32_t first_task) {
+                                                        uint32_t first_task, uint32_t bulk_bucket,
+                                                        uint32_t* __restrict__ bulk_below, uint32_t* __restrict__ bulk_act) {
     __shared__ uint32_t base[3 * kPrepThreads];
     __shared__ uint32_t wsum[kPrepThreads / 64];
     const int tid = threadIdx.x;
     const int32_t j = int32_t(blockIdx.x);
     if (j == 0 && tid == 0) {
-        *head = first_task;
-        unsigned long long* resv = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned char*>(head) + 128);
+        unsigned long long* resv = uw_queue_resv(head);
         resv[0] = 0ull;
         resv[kResvIndsOffset] = 0ull;
         resv[kExitOffset] = 0ull;
@@ -265,6 +258,15 @@ __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restric
     base[3 * tid + 1] = off + tot[0] + before[1];
     base[3 * tid + 2] = off + tot[0] + tot[1] + before[2];
     __syncthreads();
+    if (j == 0 && tid == 0) {  // workgroup 0 has no workgroups before it: base[] are the global offsets
+        const uint32_t nbig = (bulk_bucket > 0u && bulk_bucket < uint32_t(kOrderBuckets)) ? base[bulk_bucket] : 0u;
+        *uw_queue_bulk_count(head) = nbig;
+        *head = nbig + first_task;
+    }
+    if (valid && bulk_below) {
+        bulk_below[c] = 0u;
+        bulk_act[c] = 0u;
+    }
     if (valid) {
         uw_task t;
         t.c = c;
@@ -283,18 +285,7 @@ __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restric
 // written by the last workgroup to leave, so no other atomic goes to a shared counter per chunk.
 __device__ __forceinline__ void reserve_ranges(const BatchArgs& a, uint32_t nv, uint32_t ni, unsigned long long* vo,
                                                unsigned long long* io, uint32_t* flags) {
-    if (a.packed) {
-        const unsigned long long old = atomicAdd(a.resv, (unsigned long long)nv | ((unsigned long long)ni << 32));
-        *vo = old & 0xFFFFFFFFull;
-        *io = old >> 32;
-    } else {
-        *vo = atomicAdd(a.resv, (unsigned long long)nv);
-        *io = atomicAdd(a.resv + kResvIndsOffset, (unsigned long long)ni);
-    }
-    if (*vo + nv > a.vcap || *io + ni > a.icap) {
-        *flags |= UW_CHUNK_OVERFLOW;
-        atomicOr(&a.totals->overflow, 1u);
-    }
+    uw_reserve_ranges(a.resv, a.packed, a.vcap, a.icap, a.totals, nv, ni, vo, io, flags);
 }
 
 // Copies the record of queue task t (c = -1 past the end of the queue) and t to shared memory.
@@ -516,7 +507,8 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     if (tid < UW_MAX_OCTAVES) s_freq[tid] = a.f.octave_freq[tid];
     const float iso = a.f.iso;
 
-    // Task queue. Workgroup g starts with task g (the queue head starts at the grid size). While
+    // Task queue. Workgroup g starts with task nbig + g (nbig: the entries taken by the bulk path;
+    // the queue head starts at nbig plus the grid size). While
     // at least two grids' worth of tasks lie beyond the current one, a one-pass task is given one
     // task of lookahead: the block's last lane, the one with the least density work, claims the
     // next task and stages its record in shared memory once its own share of the density
@@ -525,7 +517,7 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     // and for multi-slab tasks (long enough that the round trips hardly count), the next task is
     // claimed when the block is free, which keeps the end of the launch balanced.
     const bool pf = tid == BLOCK - 1;
-    if (pf) stage_task(a, blockIdx.x, s_next);
+    if (pf) stage_task(a, *uw_queue_bulk_count(a.head) + blockIdx.x, s_next);
     uint32_t nonblank = 0;  // thread 0: chunks of this workgroup with indices
     bool staged = true;  // block-uniform: s_next holds the next task
 
@@ -946,11 +938,12 @@ void uw_default_batch_tuning(uw_batch_tuning* t) {
         if (L * L * L * 4 > kDensBudget) sx = max(1, kDensBudget / (4 * L * L) - 1);
         t->slab[n] = int8_t(min(sx, n));
     }
+    t->bulk_min = kDefaultBulkMin;
 }
 
 int uw_check_batch_tuning(const uw_batch_tuning& t) {
     if (t.block != 128 && t.block != 192 && t.block != 256 && t.block != 512) return UW_ERR_INVALID_ARG;
-    if (t.grid < 0 || t.max_res < 1 || t.max_res > kMaxN) return UW_ERR_INVALID_ARG;
+    if (t.grid < 0 || t.max_res < 1 || t.max_res > kMaxN || t.bulk_min < 0) return UW_ERR_INVALID_ARG;
     for (int n = 1; n <= kMaxN; ++n)
         if (t.slab[n] < 1 || t.slab[n] > n) return UW_ERR_INVALID_ARG;
     return UW_OK;
@@ -961,6 +954,8 @@ namespace {
 // Everything a batched launch needs that must not happen inside stream capture: buffer
 // allocation, the kernel's LDS attribute and the persistent grid size.
 struct BatchLaunch {
+    uw_batch_tuning pt;  // the persistent kernel's shape: max_res below bulk_min when the bulk path is on
+    uw_bulk_shape bulk;
     BatchedKernel k;
     size_t lds;
     int grid;
@@ -971,10 +966,16 @@ struct BatchLaunch {
 };
 
 // Launch shape of a batched launch (kernel, LDS, grid, ring, scratch sizes); allocates nothing.
-int shape_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
-    const uw_batch_tuning& tune = ctx->batch_tuning;
+// allow_bulk = false keeps every chunk on the persistent kernel.
+int shape_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl, bool allow_bulk = true) {
     const int32_t octaves = ctx->params.octaves;
-    bl->ztab_bytes = size_t(nchunks) * uw_ztab_bytes(octaves, tune.max_res + 1);
+    bl->ztab_bytes = size_t(nchunks) * uw_ztab_bytes(octaves, ctx->batch_tuning.max_res + 1);
+    memset(&bl->bulk, 0, sizeof(bl->bulk));
+    if (allow_bulk)
+        if (int rc = uw_bulk_shape_for(ctx, ctx->batch_tuning, nchunks, &bl->bulk)) return rc;
+    bl->pt = ctx->batch_tuning;
+    if (bl->bulk.on) bl->pt.max_res = max(1, ctx->batch_tuning.bulk_min - 1);
+    const uw_batch_tuning& tune = bl->pt;
     bl->k = pick_kernel(tune.block, octaves);
     const int32_t list_cap = list_cap_for(tune);
     const bool one_pass = one_pass_only(tune, list_cap);
@@ -1041,13 +1042,21 @@ int prepare_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl) {
         UW_HIP(ctx, hipMalloc(&ctx->d_batch_dscr, bl->dscr_bytes));
         ctx->batch_dscr_bytes = bl->dscr_bytes;
     }
+    if (bl->bulk.bytes > ctx->batch_bulk_bytes) {
+        (void)hipFree(ctx->d_batch_bulk);
+        ctx->d_batch_bulk = nullptr;
+        ctx->batch_bulk_bytes = 0;
+        UW_HIP(ctx, hipMalloc(&ctx->d_batch_bulk, bl->bulk.bytes));
+        ctx->batch_bulk_bytes = bl->bulk.bytes;
+    }
     return UW_OK;
 }
 
 int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, const int32_t* res, int32_t nchunks,
                    const uw_mesh_out* out, uint32_t* head, void* task_buf, unsigned char* ztab, float* dscr,
-                   hipStream_t st, uw_vertex_tape* tape = nullptr) {
-    const uw_batch_tuning& tune = ctx->batch_tuning;
+                   unsigned char* bulk_scratch, hipStream_t st, uw_vertex_tape* tape = nullptr) {
+    const uw_batch_tuning& tune = bl.pt;
+    const int32_t max_res = ctx->batch_tuning.max_res;  // z tables and reservations cover bulk chunks too
     BatchArgs a;
     memset(&a, 0, sizeof(a));
     a.f = uw_make_field(ctx->params);
@@ -1072,14 +1081,14 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     a.tasks = tasks;
     a.resv = reinterpret_cast<unsigned long long*>(reinterpret_cast<unsigned char*>(head) + 128);
     // packed reservations need every launch total below 2^32 (chunks above max_res emit nothing)
-    const uint64_t worst_v = uint64_t(nchunks) * uw_max_verts(tune.max_res),
-                   worst_i = uint64_t(nchunks) * uw_max_inds(tune.max_res);
+    const uint64_t worst_v = uint64_t(nchunks) * uw_max_verts(max_res),
+                   worst_i = uint64_t(nchunks) * uw_max_inds(max_res);
     a.packed = (worst_v < (1ull << 32) && worst_i < (1ull << 32)) ? 1 : 0;
     a.head = head;
     a.ztab = ztab;
     a.dscr = bl.dscr_stride ? dscr : nullptr;
     a.dscr_stride = bl.dscr_stride;
-    a.zstride = tune.max_res + 1;
+    a.zstride = max_res + 1;
     const uint64_t zblocks = (uint64_t(nchunks) * uint64_t(a.zstride + 1) + kPrepThreads - 1) / kPrepThreads;
     const int32_t nest = order_blocks(nchunks);
     if (zblocks + uint64_t(nest) > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
@@ -1089,12 +1098,33 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     if (int rc = uw_prof_start(ctx, st, UW_STAGE_BATCH_SETUP)) return rc;
     hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks) + uint32_t(nest)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords,
                        res, nchunks, keys, hist, nest, a.zstride, ztab);
+    uw_bulk_io bio;
+    memset(&bio, 0, sizeof(bio));
+    uint32_t bulk_bucket = 0;
+    uint32_t *bulk_below = nullptr, *bulk_act = nullptr;
+    if (bl.bulk.on) {
+        bulk_bucket = uint32_t(kMaxN - ctx->batch_tuning.bulk_min + 1) * kSurfBins;
+        bio.tasks = tasks;
+        bio.qhead = head;
+        bio.ztab = ztab;
+        bio.zstride = a.zstride;
+        bio.packed = a.packed;
+        bio.out = out;
+        bio.tape = tape;
+        bio.scratch = bulk_scratch;
+        bulk_below = reinterpret_cast<uint32_t*>(bulk_scratch + bl.bulk.below_off);
+        bulk_act = reinterpret_cast<uint32_t*>(bulk_scratch + bl.bulk.act_off);
+    }
     hipLaunchKernelGGL(k_place, dim3(uint32_t(nest)), dim3(kPrepThreads), 0, st, coords, res, nchunks, keys, hist, nest,
-                       tasks, head, a.totals, uint32_t(bl.grid));
+                       tasks, head, a.totals, uint32_t(bl.grid), bulk_bucket, bulk_below, bulk_act);
     if (int rc = uw_prof_end(ctx, st)) return rc;
+    // bulk chunks: densities, counts and range reservations before the persistent kernel (whose
+    // last workgroup writes the launch totals from the shared range counter), emission after it
+    if (int rc = uw_bulk_launch_front(ctx, bl.bulk, bio, st)) return rc;
     if (int rc = uw_prof_start(ctx, st)) return rc;
     hipLaunchKernelGGL(bl.k, dim3(uint32_t(bl.grid)), dim3(uint32_t(tune.block)), bl.lds, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
+    if (int rc = uw_bulk_launch_back(ctx, bl.bulk, bio, st)) return rc;
     UW_HIP(ctx, hipGetLastError());
     return UW_OK;
 }
@@ -1124,7 +1154,7 @@ int uw_build_chunks_batched(uw_ctx* ctx, const int32_t* coords, const int32_t* r
         return UW_OK;
     }
     return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, ctx->d_batch_ztab,
-                          ctx->d_batch_dscr, st);
+                          ctx->d_batch_dscr, ctx->d_batch_bulk, st);
 }
 
 int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
@@ -1143,7 +1173,7 @@ int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32
         return UW_OK;
     }
     return launch_batched(ctx, bl, coords, res, nchunks, out, ctx->d_batch_head, ctx->d_batch_order, ctx->d_batch_ztab,
-                          ctx->d_batch_dscr, st, tape);
+                          ctx->d_batch_dscr, ctx->d_batch_bulk, st, tape);
 }
 
 int uw_get_batch_tuning(const uw_ctx* ctx, uw_batch_tuning* t) {
@@ -1169,14 +1199,16 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     if (int rc = prepare_batched(ctx, nchunks, &bl)) return rc;
     // the graph owns its queue state, task records and z tables (the context's may be reallocated):
     // queue state at byte 0, task records (with the ordering keys and histograms) after it, z tables
-    // after them
+    // after them, then the x-slab density scratch and the bulk path's scratch
     uint32_t* gbuf = nullptr;
     const size_t ord_bytes = (kQueueBytes + order_bufs(nchunks).bytes + 255) & ~size_t(255);
     const size_t ztab_bytes = (bl.ztab_bytes + 255) & ~size_t(255);
-    UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + ztab_bytes + bl.dscr_bytes));
+    const size_t dscr_bytes = (bl.dscr_bytes + 255) & ~size_t(255);
+    UW_HIP(ctx, hipMalloc(&gbuf, ord_bytes + ztab_bytes + dscr_bytes + bl.bulk.bytes));
     unsigned char* gtasks = reinterpret_cast<unsigned char*>(gbuf) + kQueueBytes;
     unsigned char* gztab = reinterpret_cast<unsigned char*>(gbuf) + ord_bytes;
     float* gdscr = reinterpret_cast<float*>(gztab + ztab_bytes);
+    unsigned char* gbulk = gztab + ztab_bytes + dscr_bytes;
     hipStream_t cs;
     if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) {
         (void)hipFree(gbuf);
@@ -1187,7 +1219,7 @@ int uw_batched_graph_create(uw_ctx* ctx, const int32_t* coords, const int32_t* r
     int rc = UW_OK;
     const bool was_profiling = ctx->profiling;
     ctx->profiling = false;  // no timing events inside a graph
-    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gtasks, gztab, gdscr, cs);
+    if (e == hipSuccess) rc = launch_batched(ctx, bl, coords, res, nchunks, out, gbuf, gtasks, gztab, gdscr, gbulk, cs);
     ctx->profiling = was_profiling;
     hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &graph) : hipSuccess;
     (void)hipStreamDestroy(cs);
@@ -1268,7 +1300,7 @@ int uw_build_chunks_batched_noalloc(uw_ctx* ctx, const int32_t* coords, const in
     if (!base) return uw_fail(ctx, UW_ERR_INVALID_ARG, "no-allocation scratch not set up on this device");
     hipStream_t st = static_cast<hipStream_t>(stream);
     BatchLaunch bl;
-    if (int rc = shape_batched(ctx, nchunks > 0 ? nchunks : 1, &bl)) return rc;
+    if (int rc = shape_batched(ctx, nchunks > 0 ? nchunks : 1, &bl, false)) return rc;
     const size_t need = noalloc_bytes(bl, nchunks > 0 ? nchunks : 1);
     if (need > kNoAllocBytes)
         return uw_fail(ctx, UW_ERR_INVALID_ARG,
@@ -1287,7 +1319,7 @@ int uw_build_chunks_batched_noalloc(uw_ctx* ctx, const int32_t* coords, const in
     if (st) UW_HIP(ctx, hipStreamIsCapturing(st, &cs));
     const bool was_profiling = ctx->profiling;
     if (cs != hipStreamCaptureStatusNone) ctx->profiling = false;
-    const int rc = launch_batched(ctx, bl, coords, res, nchunks, out, head, tasks, ztab, nullptr, st);
+    const int rc = launch_batched(ctx, bl, coords, res, nchunks, out, head, tasks, ztab, nullptr, nullptr, st);
     ctx->profiling = was_profiling;
     return rc;
 }

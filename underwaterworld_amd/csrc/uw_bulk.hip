@@ -1,0 +1,671 @@
+// Bulk path of the batched build: chunks of at least `bulk_min` cells per axis (the 24^3 and 32^3
+// chunks of mixed-resolution and adjoint launches) leave the persistent queue. One workgroup per
+// chunk would run a 24^3 chunk's density, classification and emission as a serial chain of x-slab
+// phases; here every phase is spread over the device instead:
+//
+//   k_bulk_density  one lane per z-line (lanes on consecutive lines t = x L + y of one chunk),
+//                   densities into a per-chunk HBM field [x][z][y], so each sample's store is a row
+//   k_bulk_count    one workgroup per (chunk, x-slab of 8 cell planes): cube cases and, per row
+//                   (x, y), the owned vertex records and triangles (chunk.rs:141-164 with the
+//                   ownership rule)
+//   k_bulk_scan     one workgroup per chunk: the chunk's row bases (first-seen order within the
+//                   chunk), its totals and record, and one reservation of its vertex and index
+//                   ranges on the launch's range counter (the persistent kernel's reservations use
+//                   the same counter, so ranges never overlap)
+//   k_bulk_emit     one workgroup per (chunk, x-slab of 4 cell planes): classifies the slab's planes
+//                   and the one below it (owners of shared records lie at most one plane back),
+//                   then writes the planes' vertices, indices and, when requested, tape, colours
+//                   and collision records
+//
+// The queue set-up (k_prep / k_place in uw_batched.hip) sorts chunks by resolution, largest first,
+// so the bulk chunks are the first `nbig` queue entries; k_place publishes nbig in the queue state
+// and starts the persistent kernel's queue after them. Output (vertex numbering, index order,
+// records) is that of the persistent and plain paths, chunk by chunk.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "uw_mesh.h"
+#include "uw_noise.h"
+#include "uw_queue.h"
+#include "uw_wave.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
+constexpr int kMaxL = kMaxN + 1;
+// Distinct x values among a density workgroup's 256 consecutive lines t = x L + y of a chunk with
+// L samples per axis: at most (256 - 1) / L + 2.
+__host__ __device__ constexpr int bulk_nx_tab(int32_t L) { return (kBlock - 1) / L + 2; }
+
+struct BulkArgs {
+    uw_field f;
+    const uw_tables* tab;
+    const uw_task* tasks;
+    uint32_t* qhead;            // queue state: bulk entry count, range and exit counters
+    int32_t max_res;            // largest resolution of the launch (sizes the per-chunk slots)
+    int32_t Lm;                 // max_res + 1
+    int32_t wpc;                // density workgroup items per chunk: ceil(Lm^2 / 256)
+    int32_t nx_tab;             // x entries per octave in the density kernel's LDS (smallest bulk L)
+    const unsigned char* ztab;  // per-chunk z tables written by k_prep
+    int32_t zstride;
+    int32_t packed;
+    float* dens;                  // [j][Lm^3], chunk layout [x][z][y] with the chunk's own L
+    unsigned long long* rows;     // [j][max_res^2]: per row (x n + y) records | triangles << 32
+    uint32_t* below;              // [j] early-blank bits, zeroed by k_place
+    uint32_t* act;                // [j] active cells, zeroed by k_place
+    unsigned long long* rec;      // [j][4]: vertex base, index base, vertices, indices (~0: nothing to emit)
+    uw_chunk_info* info;
+    uw_totals* totals;
+    float* verts;
+    uint32_t* inds;
+    uw_vertex_tape* tape;
+    uw_emit_opts eo;
+    uint64_t vcap, icap;
+};
+
+__device__ __forceinline__ uint32_t bulk_count(const BulkArgs& a) { return *uw_queue_bulk_count(a.qhead); }
+
+// Inclusive scan within each 32-lane half of a wave.
+__device__ __forceinline__ uint32_t half_incl_scan(uint32_t v) {
+    v += uw_dpp_u32<UW_DPP_ROW_SHR(1), 0xF>(v);
+    v += uw_dpp_u32<UW_DPP_ROW_SHR(2), 0xF>(v);
+    v += uw_dpp_u32<UW_DPP_ROW_SHR(4), 0xF>(v);
+    v += uw_dpp_u32<UW_DPP_ROW_SHR(8), 0xF>(v);
+    v += uw_dpp_u32<UW_DPP_ROW_BCAST15, 0xA>(v);
+    return v;
+}
+
+// Corner densities of cell (y, z) from two LDS sample planes [z][y] of pitch L (planes x, x + 1).
+__device__ __forceinline__ void plane_corners(const float* p0, const float* p1, int32_t L, int32_t y, int32_t z,
+                                              float r[8]) {
+    const int o = z * L + y;
+    r[0] = p0[o];
+    r[1] = p1[o];
+    r[2] = p1[o + L];
+    r[3] = p0[o + L];
+    r[4] = p0[o + 1];
+    r[5] = p1[o + 1];
+    r[6] = p1[o + L + 1];
+    r[7] = p0[o + L + 1];
+}
+
+__device__ __forceinline__ void load_task(const BulkArgs& a, uint32_t j, int32_t* c, int32_t* n, int32_t* cx,
+                                          int32_t* cy, int32_t* cz) {
+    const int4* p = reinterpret_cast<const int4*>(a.tasks + j);
+    const int4 r0 = p[0];
+    const int4 r1 = p[1];
+    *c = __builtin_amdgcn_readfirstlane(r0.x);
+    *n = __builtin_amdgcn_readfirstlane(r0.y);
+    *cx = __builtin_amdgcn_readfirstlane(r0.z);
+    *cy = __builtin_amdgcn_readfirstlane(r0.w);
+    *cz = __builtin_amdgcn_readfirstlane(r1.x);
+}
+
+// ---------------------------------------------------------------------------------------------
+template <int OCT>
+__global__ void __launch_bounds__(kBlock) k_bulk_density(BulkArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int32_t O = uw_octaves<OCT>::count(a.f);
+    float* G3 = reinterpret_cast<float*>(smem);
+    uint8_t* P = reinterpret_cast<uint8_t*>(G3 + 3 * 256);
+    uw_axis* AYs = reinterpret_cast<uw_axis*>(P + 256);  // [O][256]: each line's y entries
+    uw_axis* AXs = AYs + O * kBlock;                     // [O][nx_tab]: the workgroup's x entries
+    __shared__ double s_freq[UW_MAX_OCTAVES];
+    __shared__ uint32_t s_bits;
+    const int tid = threadIdx.x;
+    for (int i = tid; i < 256; i += kBlock) {
+        const uint8_t p = a.tab->perm[i];
+        P[i] = p;
+        const float4 g = a.tab->grad[p & 15];
+        G3[3 * i] = g.x;
+        G3[3 * i + 1] = g.y;
+        G3[3 * i + 2] = g.z;
+    }
+    if (tid < UW_MAX_OCTAVES) s_freq[tid] = a.f.octave_freq[tid];
+    const uint32_t items = bulk_count(a) * uint32_t(a.wpc);
+    const int32_t rw = OCT > 0 ? uw_zrec_words(OCT) : uw_zrec_words(O);
+    const uint64_t slot = uint64_t(a.Lm) * a.Lm * a.Lm;
+    for (uint32_t item = blockIdx.x; item < items; item += gridDim.x) {
+        const uint32_t j = item / uint32_t(a.wpc), wb = item - j * uint32_t(a.wpc);
+        int32_t c, n, cx, cy, cz;
+        load_task(a, j, &c, &n, &cx, &cy, &cz);
+        if (n < 1 || n > a.max_res) continue;
+        const int32_t L = n + 1, lines = L * L, t0 = int32_t(wb) * kBlock;
+        if (t0 >= lines) continue;
+        __syncthreads();  // the previous item's tables are read no more
+        const int32_t t = t0 + tid;
+        const bool valid = t < lines;
+        const int32_t xfirst = t0 / L;
+        const int32_t x = valid ? t / L : xfirst, y = valid ? t - x * L : 0;
+        const int32_t nx = min(t0 + kBlock - 1, lines - 1) / L - xfirst + 1;
+        const float step = float(a.f.extent) / float(n);
+        if (tid == 0) s_bits = 0u;
+        for (int k = 0; k < O; ++k) AYs[k * kBlock + tid] = uw_axis_of(uw_sample_coord(a.f, 1, cy, y, step), s_freq[k]);
+        for (int e = tid; e < O * nx; e += kBlock) {
+            const int k = e / nx, jx = e - k * nx;
+            AXs[k * a.nx_tab + jx] = uw_hash_x(uw_axis_of(uw_sample_coord(a.f, 0, cx, xfirst + jx, step), s_freq[k]), P);
+        }
+        __syncthreads();
+        const uw_ztab Zt = uw_ztab_of(a.ztab, c, O, a.zstride);
+        const uw_axis* ax = AXs + (x - xfirst);
+        const uw_axis* ay = AYs + tid;
+        float* out = a.dens + uint64_t(j) * slot + (size_t(x) * L) * L + y;
+        float vmin = INFINITY, vmax = -INFINITY;
+        const int32_t nxt = a.nx_tab;
+        uw_noise_zline<OCT, false>(
+            a.f, [=](int k) { return ax[k * nxt]; }, [=](int k) { return ay[k * kBlock]; }, P, G3, Zt.rec, rw, L,
+            vmin, vmax, [=](int32_t iz, float rho) {
+                if (valid) out[iz * L] = rho;
+            });
+        const int bits = valid ? uw_below_bits(vmin, vmax, a.f.iso) : 0;
+        if (bits) atomicOr(&s_bits, uint32_t(bits));
+        __syncthreads();
+        if (tid == 0 && s_bits) atomicOr(a.below + j, s_bits);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Count and emit work on x-slabs: workgroup b takes chunk entry b / slabs and the cell planes
+// [X0, X1) of slab b % slabs (one launch covers every chunk of the launch; workgroups past the bulk
+// entries or the chunk's planes leave at once). Sample planes pass through a ring of two LDS
+// planes; the next plane is fetched into registers while the current one is classified, so the
+// slab's plane loads overlap its work.
+constexpr int kCountPlanes = 8;  // cell planes per count workgroup
+constexpr int kPlaneRegs = (kMaxL * kMaxL + kBlock - 1) / kBlock;
+
+struct PlaneRegs {
+    float v[kPlaneRegs];
+};
+__device__ __forceinline__ void plane_fetch(const float* __restrict__ src, int32_t LL, PlaneRegs& r) {
+#pragma unroll
+    for (int i = 0; i < kPlaneRegs; ++i) {
+        const int e = int(threadIdx.x) + i * kBlock;
+        r.v[i] = e < LL ? src[e] : 0.0f;
+    }
+}
+__device__ __forceinline__ void plane_put(float* dst, int32_t LL, const PlaneRegs& r) {
+#pragma unroll
+    for (int i = 0; i < kPlaneRegs; ++i) {
+        const int e = int(threadIdx.x) + i * kBlock;
+        if (e < LL) dst[e] = r.v[i];
+    }
+}
+
+struct BulkSlab {
+    int32_t c, n, cx, cy, cz, X0, X1;
+    uint32_t j;
+};
+// The workgroup's chunk entry and slab; false when it has nothing to do.
+__device__ __forceinline__ bool bulk_slab(const BulkArgs& a, int32_t planes, BulkSlab* s) {
+    const uint32_t slabs = uint32_t((a.max_res + planes - 1) / planes);
+    s->j = blockIdx.x / slabs;
+    if (s->j >= bulk_count(a)) return false;
+    load_task(a, s->j, &s->c, &s->n, &s->cx, &s->cy, &s->cz);
+    if (s->n < 1 || s->n > a.max_res) return false;
+    s->X0 = int32_t(blockIdx.x - s->j * slabs) * planes;
+    s->X1 = min(s->n, s->X0 + planes);
+    return s->X0 < s->X1;
+}
+
+// Per-row counts of one x-slab of a chunk: rows[j][x n + y] = owned records | triangles << 32. The
+// slab's kCountPlanes + 1 sample planes are fetched into registers at once (one memory round trip
+// per workgroup) and pass through a ring of two LDS planes.
+__global__ void __launch_bounds__(kBlock) k_bulk_count(BulkArgs a) {
+    __shared__ float SP[2][kMaxL * kMaxL];
+    __shared__ uint32_t LUT[256];
+    BulkSlab s;
+    if (!bulk_slab(a, kCountPlanes, &s)) return;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t n = s.n, L = n + 1, LL = L * L;
+    const float* src = a.dens + uint64_t(s.j) * uint64_t(a.Lm) * a.Lm * a.Lm;
+    PlaneRegs pr[kCountPlanes + 1];
+#pragma unroll
+    for (int p = 0; p <= kCountPlanes; ++p)
+        if (s.X0 + p <= s.X1) plane_fetch(src + int64_t(s.X0 + p) * LL, LL, pr[p]);
+    for (int i = tid; i < 256; i += kBlock) LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(a.tab->tri[i] & 15u) << 16);
+    plane_put(SP[0], LL, pr[0]);
+    plane_put(SP[1], LL, pr[1]);
+    __syncthreads();
+    unsigned long long* rows = a.rows + uint64_t(s.j) * a.max_res * a.max_res;
+    const float iso = a.f.iso;
+    uint32_t act = 0;
+#pragma unroll
+    for (int p = 0; p < kCountPlanes; ++p) {
+        const int32_t x = s.X0 + p;
+        if (x < s.X1) {
+            const float* p0 = SP[p & 1];
+            const float* p1 = SP[(p + 1) & 1];
+            for (int32_t q = wave; 2 * q < n; q += kBlock / 64) {
+                const int32_t y = 2 * q + (lane >> 5), z = lane & 31;
+                uint32_t v = 0;
+                if (y < n && z < n) {
+                    float r[8];
+                    plane_corners(p0, p1, L, y, z, r);
+                    const uint32_t cs = uw_case_of(r, iso);
+                    const uint32_t w = LUT[cs];
+                    v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | ((w >> 16) << 10) |
+                        ((cs != 0u && cs != 255u) ? (1u << 20) : 0u);
+                }
+                const uint32_t inc = half_incl_scan(v);
+                if ((lane & 31) == 31 && y < n) {
+                    rows[x * n + y] = (unsigned long long)(inc & 0x3FFu) | ((unsigned long long)((inc >> 10) & 0x3FFu) << 32);
+                    act += inc >> 20;
+                }
+            }
+            if (p + 2 <= kCountPlanes && x + 1 < s.X1) {
+                __syncthreads();  // sample plane x is read no more: its slot takes plane x + 2
+                plane_put(SP[p & 1], LL, pr[p + 2]);
+                __syncthreads();
+            }
+        }
+    }
+    if (act) atomicAdd(a.act + s.j, act);
+}
+
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long bulk_block_scan(unsigned long long v, unsigned long long* total) {
+    __shared__ unsigned long long ws[kBlock / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const unsigned long long inc = uw_wave_incl_scan(v);
+    if (lane == 63) ws[wave] = inc;
+    __syncthreads();
+    unsigned long long base = 0, sum = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) {
+        base += w < wave ? ws[w] : 0ull;
+        sum += ws[w];
+    }
+    __syncthreads();
+    *total = sum;
+    return base + inc - v;
+}
+
+__global__ void __launch_bounds__(kBlock) k_bulk_scan(BulkArgs a) {
+    const int tid = threadIdx.x;
+    const uint32_t nbig = bulk_count(a);
+    unsigned long long* resv = uw_queue_resv(a.qhead);
+    constexpr int kPer = (kMaxN * kMaxN + kBlock - 1) / kBlock;  // rows per thread
+    for (uint32_t j = blockIdx.x; j < nbig; j += gridDim.x) {
+        int32_t c, n, cx, cy, cz;
+        load_task(a, j, &c, &n, &cx, &cy, &cz);
+        unsigned long long* rec = a.rec + 4 * uint64_t(j);
+        if (n < 1 || n > a.max_res) {
+            if (tid == 0) {
+                uw_chunk_info r = {UW_CHUNK_INVALID | UW_CHUNK_BLANK, 0, 0, 0, 0, 0};
+                a.info[c] = r;
+                rec[0] = ~0ull;
+            }
+            continue;
+        }
+        const int32_t R = n * n;
+        unsigned long long* rows = a.rows + uint64_t(j) * a.max_res * a.max_res;
+        unsigned long long v[kPer];
+        unsigned long long s = 0;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int r = tid * kPer + k;
+            v[k] = r < R ? rows[r] : 0ull;
+            s += v[k];
+        }
+        unsigned long long tot;
+        unsigned long long ex = bulk_block_scan(s, &tot);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int r = tid * kPer + k;
+            if (r < R) rows[r] = ex;
+            ex += v[k];
+        }
+        if (tid == 0) {
+            const uint32_t nv = uint32_t(tot & 0xFFFFFFFFull), ni = 3u * uint32_t(tot >> 32);
+            uint32_t flags = 0;
+            unsigned long long vo = 0, io = 0;
+            if (!(a.below[j] & UW_BELOW_NOT_ABOVE)) flags = UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP;
+            else if (ni == 0) flags = UW_CHUNK_BLANK;
+            if (ni > 0) uw_reserve_ranges(resv, a.packed, a.vcap, a.icap, a.totals, nv, ni, &vo, &io, &flags);
+            if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
+            uw_chunk_info r = {flags, nv, ni, a.act[j], vo, io};
+            a.info[c] = r;
+            const bool emit = ni > 0 && !(flags & UW_CHUNK_OVERFLOW);
+            rec[0] = emit ? vo : ~0ull;
+            rec[1] = io;
+            rec[2] = nv;
+            rec[3] = ni;
+            if (ni > 0) atomicAdd(resv + kExitOffset, 1ull << 32);  // non-blank chunks of the launch
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Emission of one x-slab of a chunk: cell planes xs = X0 - 1 (when X0 > 0: classified for owner
+// lookups only) to X1 - 1. The slab's sample planes and row bases are fetched at once (one memory
+// round trip per workgroup). Per emitted plane the active cells are compacted into a list, then one
+// lane per listed cell writes its vertices and one lane per triangle its three index entries.
+constexpr int kT2C = 2048;  // triangles of a plane taken one per lane (more: one lane per cell)
+
+// P: cell planes emitted per workgroup (the host takes fewer for launches of few chunks, whose
+// workgroups would otherwise not fill the device).
+template <int P>
+__global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
+    constexpr int kEmitPlanes = P;
+    __shared__ float SP[2][kMaxL * kMaxL];     // sample planes x, x + 1 ([z][y], pitch L)
+    __shared__ uint32_t CB[2][kMaxN * kMaxN];  // chunk-local first vertex id: cells of planes x - 1, x
+    __shared__ uint8_t CC[2][kMaxN * kMaxN];   // cube cases of the same cells
+    // active cells of plane x: y 32 + z | plane-local triangle prefix << 10 | row-local triangle
+    // prefix << 23 (the cell's index entries start at 3 (row's triangle base + row-local prefix))
+    __shared__ uint32_t LIST[kMaxN * kMaxN];
+    __shared__ uint16_t T2C[kT2C];             // plane-local triangle -> list slot
+    __shared__ uint64_t TRI[256];
+    __shared__ uint32_t LUT[256];
+    __shared__ uint16_t BEF[256 * 12];
+    __shared__ unsigned long long RB[(kEmitPlanes + 1) * kMaxN];  // row bases of cell planes xs .. X1 - 1
+    __shared__ uint32_t s_cnt;  // listed cells | triangles << 16
+    BulkSlab s;
+    if (!bulk_slab(a, kEmitPlanes, &s)) return;
+    const unsigned long long* rec = a.rec + 4 * uint64_t(s.j);
+    const unsigned long long vo = rec[0];
+    if (vo == ~0ull) return;  // nothing to emit (blank, overflow or invalid chunk)
+    const unsigned long long io = rec[1], vlim = vo + rec[2], ilim = io + rec[3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t n = s.n, L = n + 1, LL = L * L;
+    const int32_t xs = s.X0 > 0 ? s.X0 - 1 : s.X0;
+    const float* src = a.dens + uint64_t(s.j) * uint64_t(a.Lm) * a.Lm * a.Lm;
+    const unsigned long long* rows = a.rows + uint64_t(s.j) * a.max_res * a.max_res;
+    PlaneRegs pr[kEmitPlanes + 2];  // sample planes xs .. X1
+#pragma unroll
+    for (int p = 0; p < kEmitPlanes + 2; ++p)
+        if (xs + p <= s.X1) plane_fetch(src + int64_t(xs + p) * LL, LL, pr[p]);
+    const int32_t nrb = (s.X1 - xs) * n;
+    if (tid < nrb) RB[(tid / n) * kMaxN + tid % n] = rows[xs * n + tid];
+    for (int i = tid; i < 256; i += kBlock) {
+        const uint64_t row = a.tab->tri[i];
+        TRI[i] = row;
+        LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(row & 15u) << 16);
+    }
+    for (int i = tid; i < 256 * 12; i += kBlock) BEF[i] = (&a.tab->before[0][0])[i];
+    if (tid == 0) s_cnt = 0u;
+    plane_put(SP[0], LL, pr[0]);
+    plane_put(SP[1], LL, pr[1]);
+    __syncthreads();
+    const float iso = a.f.iso;
+    const uw_emit_opts& eo = a.eo;
+    const bool coll = eo.tri_normals || eo.tri_cells;
+    const float step = float(a.f.extent) / float(n);
+    const float ox = float(a.f.extent * s.cx), oy = float(a.f.extent * s.cy), oz = float(a.f.extent * s.cz);
+#pragma unroll
+    for (int pl = 0; pl <= kEmitPlanes; ++pl) {
+        const int32_t x = xs + pl;
+        if (x >= s.X1) break;  // block-uniform
+        const float* p0 = SP[pl & 1];
+        const float* p1 = SP[(pl + 1) & 1];
+        const bool emit_plane = x >= s.X0;
+        // classification of plane x and, when it is emitted, the list of its active cells
+        for (int32_t q = wave; 2 * q < n; q += kBlock / 64) {
+            const int32_t y = 2 * q + (lane >> 5), z = lane & 31;
+            const bool in = y < n && z < n;
+            uint32_t v = 0, cs = 0;
+            if (in) {
+                float r[8];
+                plane_corners(p0, p1, L, y, z, r);
+                cs = uw_case_of(r, iso);
+                const uint32_t w = LUT[cs];
+                const uint32_t nt = w >> 16;
+                // records (< 2^10), triangles (< 2^10) and active cells (< 2^6) of a row of 32
+                v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | (nt << 10) | ((nt ? 1u : 0u) << 20);
+            }
+            const uint32_t inc = half_incl_scan(v);
+            uint32_t old = 0;
+            if (emit_plane && (lane & 31) == 31 && y < n && (inc >> 20))
+                old = atomicAdd(&s_cnt, (inc >> 20) | (((inc >> 10) & 0x3FFu) << 16));
+            const uint32_t lbase = (lane & 32) ? uint32_t(__builtin_amdgcn_readlane(int(old), 63))
+                                               : uint32_t(__builtin_amdgcn_readlane(int(old), 31));
+            if (in) {
+                const unsigned long long rb = RB[pl * kMaxN + y];
+                const uint32_t ex = inc - v;
+                const int o = y * kMaxN + z;
+                CB[x & 1][o] = uint32_t(rb) + (ex & 0x3FFu);
+                CC[x & 1][o] = uint8_t(cs);
+                if (emit_plane && (v >> 20)) {
+                    const uint32_t sl = (lbase & 0xFFFFu) + (ex >> 20);
+                    LIST[sl] = uint32_t(o) | (((lbase >> 16) + ((ex >> 10) & 0x3FFu)) << 10) |
+                               (((ex >> 10) & 0xFFu) << 23);
+                }
+            }
+        }
+        __syncthreads();
+        if (emit_plane) {
+            const uint32_t cnt = s_cnt;
+            const uint32_t na = cnt & 0xFFFFu, ntp = cnt >> 16;
+            const bool flat = ntp <= uint32_t(kT2C);  // block-uniform
+            // chunk-local id of the record that cell (x, y, z) sees as edge ed: its own rank, or its owner's
+            auto edge_id = [&](int32_t y, int32_t z, uint32_t cs, uint32_t owned, uint32_t myv, uint32_t ed) -> uint32_t {
+                if (owned & (1u << ed)) return myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
+                int32_t qx, qy, qz;
+                uint32_t qe;
+                uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
+                const int qo = qy * kMaxN + qz;
+                return CB[qx & 1][qo] + __popc(uint32_t(BEF[CC[qx & 1][qo] * 12 + qe]) & uw_owned_mask(qx, qy, qz));
+            };
+            // index entries (and collision records) of triangle tj of a listed cell; gi: its first entry
+            auto emit_tri = [&](int32_t y, int32_t z, uint32_t cs, uint32_t owned, uint32_t myv, uint64_t row,
+                                uint32_t tj, uint64_t gi, const float* r) {
+                float q[3][3];
+#pragma unroll
+                for (uint32_t k = 0; k < 3; ++k) {
+                    const uint32_t ed = uint32_t(row >> (4 + 4 * (3 * tj + k))) & 15u;
+                    const uint32_t id = edge_id(y, z, cs, owned, myv, ed);
+                    if (gi + k < ilim) uw_store_index(a.inds, eo.ind16, gi + k, id);
+                    if (coll) uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, q[k]);
+                }
+                if (coll && gi + 2 < ilim) {
+                    const uint64_t t = gi / 3;
+                    if (eo.tri_normals) {
+                        float nm[3];
+                        uw_tri_normal(q[0], q[1], q[2], nm);
+                        eo.tri_normals[3 * t] = nm[0];
+                        eo.tri_normals[3 * t + 1] = nm[1];
+                        eo.tri_normals[3 * t + 2] = nm[2];
+                    }
+                    if (eo.tri_cells) eo.tri_cells[t] = uint32_t((x * n + y) * n + z);
+                }
+            };
+            // first index entry of a listed cell: from its row's triangle base and row-local prefix
+            auto first_index = [&](uint32_t ent, int32_t y) -> uint64_t {
+                return io + 3u * (uint32_t(RB[pl * kMaxN + y] >> 32) + (ent >> 23));
+            };
+            for (uint32_t k = tid; k < na; k += kBlock) {
+                const uint32_t ent = LIST[k];
+                const int32_t y = int32_t((ent & 0x3FFu) >> 5), z = int32_t(ent & 31u);
+                const int o = y * kMaxN + z;
+                const uint32_t cs = CC[x & 1][o];
+                float r[8];
+                plane_corners(p0, p1, L, y, z, r);
+                const uint32_t owned = uw_owned_mask(x, y, z);
+                const uint32_t myv = CB[x & 1][o];
+                for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
+                    const uint32_t ed = uint32_t(__builtin_ctz(vm));
+                    const uint64_t id = vo + myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
+                    if (id >= vlim) continue;
+                    float p[3];
+                    uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
+                    float* dst = a.verts + eo.vstride * id;
+                    dst[0] = p[0];
+                    dst[1] = p[1];
+                    dst[2] = p[2];
+                    if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
+                    if (a.tape) {
+                        const uint32_t ia = uw_edge_a(ed), ib = uw_edge_b(ed);
+                        const uint32_t ca = uw_corner_ofs(ia), cbo = uw_corner_ofs(ib);
+                        uw_vertex_tape tv;
+                        tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L + z +
+                                        int32_t((ca >> 2) & 1u));
+                        tv.b = uint32_t(((x + int32_t(cbo & 1u)) * L + y + int32_t((cbo >> 1) & 1u)) * L + z +
+                                        int32_t((cbo >> 2) & 1u));
+                        tv.rho_a = r[ia];
+                        tv.rho_b = r[ib];
+                        a.tape[id] = tv;
+                    }
+                }
+                const uint32_t ntri = LUT[cs] >> 16;
+                if (flat) {
+                    const uint32_t tp = (ent >> 10) & 0x1FFFu;
+                    for (uint32_t i = 0; i < ntri; ++i) T2C[tp + i] = uint16_t(k);
+                } else {
+                    const uint64_t row = TRI[cs];
+                    const uint64_t gi = first_index(ent, y);
+                    for (uint32_t tj = 0; tj < ntri; ++tj) emit_tri(y, z, cs, owned, myv, row, tj, gi + 3 * tj, r);
+                }
+            }
+            if (flat) {
+                __syncthreads();  // the triangle map is complete
+                for (uint32_t t = tid; t < ntp; t += kBlock) {
+                    const uint32_t k = T2C[t];
+                    const uint32_t ent = LIST[k];
+                    const int32_t y = int32_t((ent & 0x3FFu) >> 5), z = int32_t(ent & 31u);
+                    const int o = y * kMaxN + z;
+                    const uint32_t cs = CC[x & 1][o];
+                    const uint32_t tj = t - ((ent >> 10) & 0x1FFFu);
+                    float r[8];
+                    if (coll) plane_corners(p0, p1, L, y, z, r);
+                    emit_tri(y, z, cs, uw_owned_mask(x, y, z), CB[x & 1][o], TRI[cs], tj, first_index(ent, y) + 3 * tj, r);
+                }
+            }
+        }
+        if (pl + 2 < kEmitPlanes + 2 && x + 1 < s.X1) {
+            __syncthreads();  // sample plane x and the plane's lists are read no more
+            if (tid == 0) s_cnt = 0u;
+            plane_put(SP[pl & 1], LL, pr[pl + 2]);  // plane x + 2
+            __syncthreads();
+        }
+    }
+}
+
+using BulkDensityKernel = void (*)(BulkArgs);
+BulkDensityKernel pick_bulk_density(int octaves) {
+    switch (octaves) {
+        case 3: return k_bulk_density<3>;
+        case 4: return k_bulk_density<4>;
+        case 6: return k_bulk_density<6>;
+        case 8: return k_bulk_density<8>;
+        default: return k_bulk_density<0>;
+    }
+}
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+int grid_for(uw_ctx* ctx, const void* k, size_t lds, int* grid) {
+    int per_cu = 0;
+    UW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, lds));
+    if (per_cu < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bulk kernel does not fit on a CU");
+    *grid = per_cu * ctx->num_cus;
+    return UW_OK;
+}
+
+}  // namespace
+
+int uw_bulk_shape_for(uw_ctx* ctx, const uw_batch_tuning& t, int32_t nchunks, uw_bulk_shape* s) {
+    memset(s, 0, sizeof(*s));
+    s->on = t.bulk_min >= 1 && t.bulk_min <= t.max_res;
+    if (!s->on) return UW_OK;
+    s->max_res = t.max_res;
+    const int32_t Lm = t.max_res + 1;
+    s->wpc = (Lm * Lm + kBlock - 1) / kBlock;
+    const int32_t O = ctx->params.octaves;
+    s->nx_tab = bulk_nx_tab(t.bulk_min + 1);  // every bulk chunk has at least bulk_min + 1 samples per axis
+    s->lds_density = 256 * 3 * sizeof(float) + 256 + sizeof(uw_axis) * size_t(O) * size_t(kBlock + s->nx_tab);
+    const BulkDensityKernel kd = pick_bulk_density(O);
+    UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(kd), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(s->lds_density)));
+    if (int rc = grid_for(ctx, reinterpret_cast<const void*>(kd), s->lds_density, &s->grid_density)) return rc;
+    if (int rc = grid_for(ctx, reinterpret_cast<const void*>(k_bulk_scan), 0, &s->grid_scan)) return rc;
+    // count and emit: one workgroup per (chunk entry, x-slab) of every chunk of the launch
+    const int64_t nc64 = nchunks > 0 ? nchunks : 1;
+    const int64_t gc = nc64 * ((t.max_res + kCountPlanes - 1) / kCountPlanes);
+    // emit: the deepest slab (fewest halo planes) whose workgroups still cover the resident slots
+    // about four times over; a launch of few chunks gets shallow slabs, so more workgroups share
+    // its chunks
+    int resident = 0;
+    if (int rc = grid_for(ctx, reinterpret_cast<const void*>(k_bulk_emit<4>), 0, &resident)) return rc;
+    s->emit_planes = 4;
+    while (s->emit_planes > 1 && nc64 * ((t.max_res + s->emit_planes - 1) / s->emit_planes) < 4 * int64_t(resident))
+        s->emit_planes /= 2;
+    const int64_t ge = nc64 * ((t.max_res + s->emit_planes - 1) / s->emit_planes);
+    if (ge > 0x7FFFFFFF) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for the bulk path");
+    s->grid_count = int(gc);
+    s->grid_emit = int(ge);
+    // scratch for every chunk of the launch (the resolutions are on the device; a launch's bulk
+    // chunks are at most all of them)
+    const size_t nc = size_t(nchunks > 0 ? nchunks : 1);
+    size_t o = 0;
+    s->dens_off = o;
+    o = align256(o + nc * size_t(Lm) * Lm * Lm * sizeof(float));
+    s->rows_off = o;
+    o = align256(o + nc * size_t(t.max_res) * t.max_res * sizeof(unsigned long long));
+    s->below_off = o;
+    o = align256(o + nc * sizeof(uint32_t));
+    s->act_off = o;
+    o = align256(o + nc * sizeof(uint32_t));
+    s->rec_off = o;
+    o = align256(o + nc * 4 * sizeof(unsigned long long));
+    s->bytes = o;
+    return UW_OK;
+}
+
+namespace {
+BulkArgs bulk_args(uw_ctx* ctx, const uw_bulk_shape& s, const uw_bulk_io& io) {
+    BulkArgs a;
+    memset(&a, 0, sizeof(a));
+    a.f = uw_make_field(ctx->params);
+    a.tab = ctx->d_tables;
+    a.tasks = io.tasks;
+    a.qhead = io.qhead;
+    a.max_res = s.max_res;
+    a.Lm = s.max_res + 1;
+    a.wpc = s.wpc;
+    a.nx_tab = s.nx_tab;
+    a.ztab = io.ztab;
+    a.zstride = io.zstride;
+    a.packed = io.packed;
+    a.dens = reinterpret_cast<float*>(io.scratch + s.dens_off);
+    a.rows = reinterpret_cast<unsigned long long*>(io.scratch + s.rows_off);
+    a.below = reinterpret_cast<uint32_t*>(io.scratch + s.below_off);
+    a.act = reinterpret_cast<uint32_t*>(io.scratch + s.act_off);
+    a.rec = reinterpret_cast<unsigned long long*>(io.scratch + s.rec_off);
+    a.info = io.out->info;
+    a.totals = io.out->totals;
+    a.verts = io.out->verts;
+    a.inds = io.out->inds;
+    a.tape = io.tape;
+    a.eo = uw_make_emit_opts(io.out);
+    a.vcap = io.out->vert_cap;
+    a.icap = io.out->ind_cap;
+    return a;
+}
+}  // namespace
+
+int uw_bulk_launch_front(uw_ctx* ctx, const uw_bulk_shape& s, const uw_bulk_io& io, hipStream_t st) {
+    if (!s.on) return UW_OK;
+    const BulkArgs a = bulk_args(ctx, s, io);
+    const BulkDensityKernel kd = pick_bulk_density(ctx->params.octaves);
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_BULK_FRONT)) return rc;
+    hipLaunchKernelGGL(kd, dim3(uint32_t(s.grid_density)), dim3(kBlock), s.lds_density, st, a);
+    hipLaunchKernelGGL(k_bulk_count, dim3(uint32_t(s.grid_count)), dim3(kBlock), 0, st, a);
+    hipLaunchKernelGGL(k_bulk_scan, dim3(uint32_t(s.grid_scan)), dim3(kBlock), 0, st, a);
+    if (int rc = uw_prof_end(ctx, st)) return rc;
+    UW_HIP(ctx, hipGetLastError());
+    return UW_OK;
+}
+
+int uw_bulk_launch_back(uw_ctx* ctx, const uw_bulk_shape& s, const uw_bulk_io& io, hipStream_t st) {
+    if (!s.on) return UW_OK;
+    const BulkArgs a = bulk_args(ctx, s, io);
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_BULK_EMIT)) return rc;
+    switch (s.emit_planes) {
+        case 1: hipLaunchKernelGGL(k_bulk_emit<1>, dim3(uint32_t(s.grid_emit)), dim3(kBlock), 0, st, a); break;
+        case 2: hipLaunchKernelGGL(k_bulk_emit<2>, dim3(uint32_t(s.grid_emit)), dim3(kBlock), 0, st, a); break;
+        default: hipLaunchKernelGGL(k_bulk_emit<4>, dim3(uint32_t(s.grid_emit)), dim3(kBlock), 0, st, a); break;
+    }
+    if (int rc = uw_prof_end(ctx, st)) return rc;
+    UW_HIP(ctx, hipGetLastError());
+    return UW_OK;
+}

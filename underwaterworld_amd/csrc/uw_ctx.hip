@@ -89,10 +89,6 @@ void build_permutation(uint32_t seed, uint8_t out[256]) {
 
 // Gradient vectors of noise 0.8.2 perlin_3d for hash & 15; codes 12..15 repeat four of the
 // twelve edge directions.
-const int8_t kGrad[16][3] = {
-    {1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
-    {0, 1, 1},  {0, -1, 1}, {0, 1, -1}, {0, -1, -1}, {1, 1, 0},  {-1, 1, 0}, {0, -1, 1}, {0, -1, -1},
-};
 
 void build_tables(uw_tables& t, const uint8_t perm[256]) {
     memset(&t, 0, sizeof(t));
@@ -115,7 +111,7 @@ void build_tables(uw_tables& t, const uint8_t perm[256]) {
     }
     memcpy(t.perm, perm, 256);
     for (int h = 0; h < 16; ++h)
-        t.grad[h] = make_float4(float(kGrad[h][0]), float(kGrad[h][1]), float(kGrad[h][2]), 0.0f);
+        t.grad[h] = make_float4(float(kUwGrad[h][0]), float(kUwGrad[h][1]), float(kUwGrad[h][2]), 0.0f);
 }
 
 bool params_valid(const uw_params* p, std::string* why) {
@@ -246,6 +242,7 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(ctx->d_batch_order);
     (void)hipFree(ctx->d_batch_ztab);
     (void)hipFree(ctx->d_batch_dscr);
+    (void)hipFree(ctx->d_batch_bulk);
     (void)hipFree(ctx->d_vol);
     (void)hipFree(ctx->d_adj_part);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);

@@ -12,6 +12,19 @@
 #include "uw_div.h"
 
 // Device-resident lookup tables, uploaded once per context.
+// Gradient vectors of the 16 hash codes (noise 0.8.2 perlin_3d gradient_dot_v), components in
+// {-1, 0, 1}.
+constexpr int8_t kUwGrad[16][3] = {
+    {1, 1, 0},  {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0}, {1, 0, 1},  {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
+    {0, 1, 1},  {0, -1, 1}, {0, 1, -1}, {0, -1, -1}, {1, 1, 0},  {-1, 1, 0}, {0, -1, 1}, {0, -1, -1},
+};
+// Component `comp` of every code, two bits per code (two's complement), code h at bits 2h.
+__host__ __device__ constexpr uint32_t uw_grad_pack(int comp) {
+    uint32_t k = 0;
+    for (int h = 0; h < 16; ++h) k |= uint32_t(kUwGrad[h][comp] & 3) << (2 * h);
+    return k;
+}
+
 struct uw_tables {
     // Packed triangulation rows (see mc_table.h).
     uint64_t tri[256];
@@ -23,8 +36,7 @@ struct uw_tables {
     uint16_t used[256];
     // Perlin permutation (noise 0.8.2 PermutationTable).
     uint8_t perm[256];
-    // Gradient vector for hash code h & 15 (noise 0.8.2 perlin_3d gradient_dot_v), components in
-    // {-1, 0, 1}; w unused.
+    // Gradient vector for hash code h & 15 (kUwGrad[h]); w unused.
     float4 grad[16];
 };
 
@@ -63,6 +75,8 @@ struct uw_ctx {
     size_t batch_ztab_bytes = 0;
     float* d_batch_dscr = nullptr;  // per-workgroup density scratch of x-slab chunks (eager batched path)
     size_t batch_dscr_bytes = 0;
+    unsigned char* d_batch_bulk = nullptr;  // bulk path scratch of the eager batched path
+    size_t batch_bulk_bytes = 0;
     double* d_adj_part = nullptr;  // adjoint backward: per-(chunk, vertex share) parameter sums
     size_t adj_part_bytes = 0;
     unsigned char* d_vol = nullptr;  // single-volume path: tables, density field, row counts, partial sums

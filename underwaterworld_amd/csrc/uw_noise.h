@@ -63,18 +63,40 @@ __device__ __forceinline__ float uw_gdot(float4 g, float x, float y, float z) {
 
 __device__ __forceinline__ float uw_lerp(float t, float a, float b) { return fmaf(t, b - a, a); }
 
-// Gradient of permutation slot j (the vector of code P[j] & 15) from a packed LDS table of three
-// floats per slot: with a 12-byte stride the slots of a gather spread over all LDS banks (a 16-byte
-// stride puts each component of every slot on the same 8 of the 32 banks).
+// Two sources of the gradient of permutation slot j (the vector of code P[j] & 15), giving the same
+// float4 (components exactly -1, 0 or 1):
+//   const float* G3: an LDS table of three floats per slot (12-byte stride, so the slots of a
+//     gather spread over all LDS banks): three LDS reads per gather, no arithmetic;
+//   uw_gcode: an LDS byte per slot, GC[j] = 2 (P[j] & 15), and the components of the 16 gradients
+//     packed two bits each (two's complement) into constants: one LDS read and six VALU
+//     operations per gather.
+// Which is faster depends on how often a line rebuilds its lattice faces: the byte codes win when
+// the octaves are fine against the sample step (faces rebuilt at most samples, so LDS traffic and
+// bank conflicts bound the loop), the float table when the loop is bound by its VALU work.
 __device__ __forceinline__ float4 uw_grad3(const float* G3, uint32_t j) {
     const float* g = G3 + 3u * j;
     return make_float4(g[0], g[1], g[2], 0.0f);
 }
 
+struct uw_gcode {
+    const uint8_t* c;
+};
+__device__ __forceinline__ float4 uw_grad3(uw_gcode g, uint32_t j) {
+    constexpr uint32_t kx = uw_grad_pack(0), ky = uw_grad_pack(1), kz = uw_grad_pack(2);
+    const uint32_t sh = g.c[j];
+    const int gx = __builtin_amdgcn_sbfe(int(kx), sh, 2u);
+    const int gy = __builtin_amdgcn_sbfe(int(ky), sh, 2u);
+    const int gz = __builtin_amdgcn_sbfe(int(kz), sh, 2u);
+    return make_float4(float(gx), float(gy), float(gz), 0.0f);
+}
+// Code table entry of permutation slot j for uw_gcode (perm_j = P[j]).
+__host__ __device__ inline uint8_t uw_gcode_of(uint8_t perm_j) { return uint8_t((perm_j & 15u) << 1); }
+
 // One octave of improved Perlin noise at the lattice cell / fractions given by X (hashed), Y, Z.
-// P: permutation table; G3: gradient of each permutation slot (uw_grad3).
+// P: permutation table; G3: gradient source (uw_grad3).
+template <class GT>
 __device__ __forceinline__ float uw_perlin_octave(const uw_axis& X, const uw_axis& Y, const uw_axis& Z,
-                                                  const uint8_t* P, const float* G3) {
+                                                  const uint8_t* P, GT G3) {
     const uint32_t a0 = X.bits & 255, a1 = X.bits >> 8;
     const uint32_t y0 = Y.bits & 255, y1 = Y.bits >> 8;
     const uint32_t z0 = Z.bits & 255, z1 = Z.bits >> 8;
@@ -140,8 +162,9 @@ __device__ __forceinline__ void uw_line_diff(uw_line_oct& s) {
 // One z face of a line's lattice cell: A = lerp_y(lerp_x(P_..)), B = lerp_y(lerp_x(Gz_..)) for the
 // face whose z lattice byte is zb. bb = b00 | b01 << 8 | b10 << 16 | b11 << 24 (second-level
 // hashes of the line's x/y corners). The face depends on the z lattice index only through zb.
-__device__ __forceinline__ void uw_line_face(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zb,
-                                             const float* G3, float* A, float* B) {
+template <class GT>
+__device__ __forceinline__ void uw_line_face(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zb, GT G3,
+                                             float* A, float* B) {
     const uint32_t b00 = bb & 255u, b01 = (bb >> 8) & 255u, b10 = (bb >> 16) & 255u, b11 = bb >> 24;
     const float4 G00 = uw_grad3(G3, b00 ^ zb), G10 = uw_grad3(G3, b10 ^ zb), G01 = uw_grad3(G3, b01 ^ zb),
                  G11 = uw_grad3(G3, b11 ^ zb);
@@ -153,8 +176,9 @@ __device__ __forceinline__ void uw_line_face(const uw_axis& X, const uw_axis& Y,
 }
 
 // Per-line, per-octave state for the lattice cell whose low/high z bytes are packed in zbits.
+template <class GT>
 __device__ __forceinline__ uw_line_oct uw_line_setup(const uw_axis& X, const uw_axis& Y, uint32_t bb, uint32_t zbits,
-                                                     const float* G3) {
+                                                     GT G3) {
     uw_line_oct s;
     uw_line_face(X, Y, bb, zbits & 255u, G3, &s.A0, &s.B0);
     uw_line_face(X, Y, bb, (zbits >> 8) & 255u, G3, &s.A1, &s.B1);
@@ -368,9 +392,9 @@ __device__ __forceinline__ uint32_t uw_full_setup_flags(int32_t octaves) {
 // discarded), `count` samples. FORCE: the first sample builds every octave's lattice cell whatever
 // its record's face flags say (a line that starts inside a longer z range). store(iz, rho)
 // receives the densities; vmin / vmax keep the running extremes.
-template <int OCT, bool FORCE, class AxisX, class AxisY, class Store>
-__device__ __forceinline__ void uw_noise_zline(const uw_field& f, AxisX AX, AxisY AY, const uint8_t* P,
-                                               const float* G3, const uint32_t* zr, int32_t rw, int32_t count,
+template <int OCT, bool FORCE, class AxisX, class AxisY, class GT, class Store>
+__device__ __forceinline__ void uw_noise_zline(const uw_field& f, AxisX AX, AxisY AY, const uint8_t* P, GT G3,
+                                               const uint32_t* zr, int32_t rw, int32_t count,
                                                float& vmin, float& vmax, Store store) {
     constexpr int OMAX = uw_octaves<OCT>::kMax;
     const int32_t O = uw_octaves<OCT>::count(f);
@@ -438,9 +462,9 @@ __device__ __forceinline__ int uw_below_bits(float vmin, float vmax, float iso) 
 // Densities of x-planes [xa, xb) of a chunk whose tables are in T, one thread per (x, y) line.
 // Plane x is written to dst + slot(x) * L * L with slot(x) = x % nslot, z fastest.
 // Returns, per thread, the OR of the UW_BELOW_* bits of its samples.
-template <int OCT>
+template <int OCT, class GT>
 __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk_tables& T, const uw_ztab& Zt,
-                                               const uint8_t* P, const float* G3, int32_t L, int32_t xa, int32_t xb,
+                                               const uint8_t* P, GT G3, int32_t L, int32_t xa, int32_t xb,
                                                float* dst, int32_t nslot, int tid, int nthr) {
     const int32_t lines = (xb - xa) * L;
     const uw_divider dL = uw_make_divider(uint32_t(L)), dslot = uw_make_divider(uint32_t(nslot));
@@ -460,9 +484,9 @@ __device__ __forceinline__ int uw_noise_planes(const uw_field& f, const uw_chunk
 
 // Octave value and gradient with respect to the octave coordinate (f32), given the per-axis
 // lattice data and fade derivatives s'(d) = 30 d^2 (d - 1)^2.
+template <class GT>
 __device__ __forceinline__ float uw_octave_grad(const uw_axis& X, const uw_axis& Y, const uw_axis& Z, float dfx,
-                                                float dfy, float dfz, const uint8_t* P, const float* G3,
-                                                float g[3]) {
+                                                float dfy, float dfz, const uint8_t* P, GT G3, float g[3]) {
     const uint32_t a0 = X.bits & 255, a1 = X.bits >> 8;
     const uint32_t y0 = Y.bits & 255, y1 = Y.bits >> 8;
     const uint32_t z0 = Z.bits & 255, z1 = Z.bits >> 8;

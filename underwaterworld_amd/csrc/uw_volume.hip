@@ -104,8 +104,8 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_density(VolArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int32_t O = uw_octaves<OCT>::count(a.f);
     const int32_t L = a.L;
-    float* G3 = reinterpret_cast<float*>(smem);
-    uint8_t* P = reinterpret_cast<uint8_t*>(G3 + 3 * 256);
+    uint8_t* GC = smem;  // gradient code of each permutation slot (uw_gcode)
+    uint8_t* P = GC + 256;
     uw_axis* AYs = reinterpret_cast<uw_axis*>(P + 256);  // [O][256] per line
     uw_axis* AXs = AYs + O * kVolBlock;                  // [O][nx_tab] per distinct x
     __shared__ uint32_t s_bits;
@@ -122,10 +122,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_density(VolArgs a) {
     for (int i = tid; i < 256; i += kVolBlock) {
         const uint8_t p = a.tab->perm[i];
         P[i] = p;
-        const float4 g = a.tab->grad[p & 15];
-        G3[3 * i] = g.x;
-        G3[3 * i + 1] = g.y;
-        G3[3 * i + 2] = g.z;
+        GC[i] = uw_gcode_of(p);
     }
     for (int k = 0; k < O; ++k) AYs[k * kVolBlock + tid] = a.axis[(size_t(O) + k) * L + y];
     for (int e = tid; e < O * a.nx_tab; e += kVolBlock) {
@@ -144,7 +141,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_density(VolArgs a) {
         float* out = a.dens + (size_t(x) * L + z0) * L + y;
         const size_t zstride = size_t(L);
         uw_noise_zline<OCT, true>(
-            a.f, [=](int k) { return ax[k * nxt]; }, [=](int k) { return ay[k * kVolBlock]; }, P, G3,
+            a.f, [=](int k) { return ax[k * nxt]; }, [=](int k) { return ay[k * kVolBlock]; }, P, uw_gcode{GC},
             a.zrec + size_t(z0) * rw, rw, z1 - z0, vmin, vmax, [=](int32_t iz, float rho) {
                 if (valid) out[size_t(iz) * zstride] = rho;
             });
@@ -783,7 +780,7 @@ int uw_build_volume(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t n, 
     if (int rc = vol_launch_tables(ctx, a, lay, st)) return rc;
     // densities
     const VolDensityKernel dk = pick_vol_density(a.f.octaves);
-    const size_t lds = 256 * 3 * sizeof(float) + 256 + sizeof(uw_axis) * size_t(a.f.octaves) * (kVolBlock + a.nx_tab);
+    const size_t lds = 256 + 256 + sizeof(uw_axis) * size_t(a.f.octaves) * (kVolBlock + a.nx_tab);
     UW_HIP(ctx, hipFuncSetAttribute(reinterpret_cast<const void*>(dk), hipFuncAttributeMaxDynamicSharedMemorySize,
                                     int(lds)));
     const uint64_t lines = uint64_t(a.L) * uint64_t(a.L);

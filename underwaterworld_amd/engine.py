@@ -471,7 +471,8 @@ def volume_required_sizes(perlin: Perlin, n: int, chunk=(0, 0, 0)) -> tuple[int,
 def get_batch_tuning(perlin: Perlin) -> dict:
     t = _lib.UwBatchTuning()
     _lib.check(_lib.lib().uw_get_batch_tuning(perlin.ctx, ctypes.byref(t)), perlin.ctx)
-    return {"block": t.block, "grid": t.grid, "max_res": t.max_res, "slab": [int(v) for v in t.slab]}
+    return {"block": t.block, "grid": t.grid, "max_res": t.max_res, "slab": [int(v) for v in t.slab],
+            "bulk_min": t.bulk_min}
 
 
 def set_batch_tuning(perlin: Perlin, tuning: dict):
@@ -481,6 +482,7 @@ def set_batch_tuning(perlin: Perlin, tuning: dict):
     t.max_res = int(tuning.get("max_res", _lib.BATCH_MAX_RESOLUTION))
     for i, v in enumerate(tuning["slab"]):
         t.slab[i] = int(v)
+    t.bulk_min = int(tuning.get("bulk_min", 0))
     _lib.check(_lib.lib().uw_set_batch_tuning(perlin.ctx, ctypes.byref(t)), perlin.ctx)
 
 
