@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     for (int i = tid; i < 256; i += kBlock) {
         const uint8_t p = a.tab->perm[i];
         P[i] = p;
-        const float4 g = a.tab->grad[p & 15];
+        const float4 g = uw_grad_vec(p & 15u);
         G3[3 * i] = g.x;
         G3[3 * i + 1] = g.y;
         G3[3 * i + 2] = g.z;

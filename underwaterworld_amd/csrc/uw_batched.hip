@@ -495,14 +495,14 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     for (int i = tid; i < 256; i += BLOCK) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
-        const float4 g = tab->grad[p & 15];
+        const float4 g = uw_grad_vec(p & 15u);
         G3[3 * i] = g.x;
         G3[3 * i + 1] = g.y;
         G3[3 * i + 2] = g.z;
         TRI[i] = tab->tri[i];
         LUT[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
     }
-    for (int i = tid; i < 256 * 12; i += BLOCK) BEF[i] = (&tab->before[0][0])[i];
+    uw_lds_load_before(BEF, tab, tid, BLOCK);
     __shared__ double s_freq[UW_MAX_OCTAVES];
     if (tid < UW_MAX_OCTAVES) s_freq[tid] = a.f.octave_freq[tid];
     const float iso = a.f.iso;

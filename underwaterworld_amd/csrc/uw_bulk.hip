@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_density(BulkArgs a) {
     for (int i = tid; i < 256; i += kBlock) {
         const uint8_t p = a.tab->perm[i];
         P[i] = p;
-        const float4 g = a.tab->grad[p & 15];
+        const float4 g = uw_grad_vec(p & 15u);
         G3[3 * i] = g.x;
         G3[3 * i + 1] = g.y;
         G3[3 * i + 2] = g.z;
@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
     __shared__ uint16_t T2C[kT2C];             // plane-local triangle -> list slot
     __shared__ uint64_t TRI[256];
     __shared__ uint32_t LUT[256];
-    __shared__ uint16_t BEF[256 * 12];
+    __shared__ __align__(16) uint16_t BEF[256 * 12];
     __shared__ unsigned long long RB[(kEmitPlanes + 1) * kMaxN];  // row bases of cell planes xs .. X1 - 1
     __shared__ uint32_t s_cnt;  // listed cells | triangles << 16
     BulkSlab s;
@@ -382,7 +382,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
         TRI[i] = row;
         LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(row & 15u) << 16);
     }
-    for (int i = tid; i < 256 * 12; i += kBlock) BEF[i] = (&a.tab->before[0][0])[i];
+    uw_lds_load_before(BEF, a.tab, tid, kBlock);
     if (tid == 0) s_cnt = 0u;
     plane_put(SP[0], LL, pr[0]);
     plane_put(SP[1], LL, pr[1]);

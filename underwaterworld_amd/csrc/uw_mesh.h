@@ -125,6 +125,24 @@ __device__ __forceinline__ void uw_tri_normal(const float a[3], const float b[3]
     }
 }
 
+// Copies the first-seen rank masks (uw_tables::before, 6144 bytes) into a 16-byte aligned LDS
+// array with 16-byte loads, every load of a thread issued before its first store (a loop of
+// 2-byte loads, each waited for before its store, costs one memory round trip per element).
+__device__ __forceinline__ void uw_lds_load_before(uint16_t* dst, const uw_tables* tab, int tid, int nthr) {
+    constexpr int kWords = 256 * 12 * 2 / 16;
+    const uint4* src = reinterpret_cast<const uint4*>(&tab->before[0][0]);
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    const int e0 = tid, e1 = tid + nthr, e2 = tid + 2 * nthr;
+    uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0, v2 = v0;
+    if (e0 < kWords) v0 = src[e0];
+    if (e1 < kWords) v1 = src[e1];
+    if (e2 < kWords) v2 = src[e2];
+    if (e0 < kWords) d[e0] = v0;
+    if (e1 < kWords) d[e1] = v1;
+    if (e2 < kWords) d[e2] = v2;
+    for (int e = tid + 3 * nthr; e < kWords; e += nthr) d[e] = src[e];
+}
+
 // Per-launch output options shared by the emit kernels.
 struct uw_emit_opts {
     uint32_t vstride;     // floats per vertex: 3 (xyz) or 6 (xyz + sRGB colour)

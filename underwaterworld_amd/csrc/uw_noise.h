@@ -89,6 +89,17 @@ __device__ __forceinline__ float4 uw_grad3(uw_gcode g, uint32_t j) {
     const int gz = __builtin_amdgcn_sbfe(int(kz), sh, 2u);
     return make_float4(float(gx), float(gy), float(gz), 0.0f);
 }
+// Gradient vector of hash code h (0..15) from the packed constants: no table load, so filling a
+// per-slot LDS table costs one memory round trip (the permutation) instead of two.
+__device__ __forceinline__ float4 uw_grad_vec(uint32_t h) {
+    constexpr uint32_t kx = uw_grad_pack(0), ky = uw_grad_pack(1), kz = uw_grad_pack(2);
+    const uint32_t sh = 2u * h;
+    const int gx = __builtin_amdgcn_sbfe(int(kx), sh, 2u);
+    const int gy = __builtin_amdgcn_sbfe(int(ky), sh, 2u);
+    const int gz = __builtin_amdgcn_sbfe(int(kz), sh, 2u);
+    return make_float4(float(gx), float(gy), float(gz), 0.0f);
+}
+
 // Code table entry of permutation slot j for uw_gcode (perm_j = P[j]).
 __host__ __device__ inline uint8_t uw_gcode_of(uint8_t perm_j) { return uint8_t((perm_j & 15u) << 1); }
 

@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(kThreads) k_noise_grid(uw_field f, const uw_ta
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
-        const float4 g = tab->grad[p & 15];
+        const float4 g = uw_grad_vec(p & 15u);
         G3[3 * i] = g.x;
         G3[3 * i + 1] = g.y;
         G3[3 * i + 2] = g.z;
@@ -229,13 +229,13 @@ __global__ void __launch_bounds__(kThreads) k_emit(float iso, int32_t extent, co
     const float ox = float(extent * coords[3 * chunk]), oy = float(extent * coords[3 * chunk + 1]),
                 oz = float(extent * coords[3 * chunk + 2]);
     __shared__ uint64_t s_tri[256];
-    __shared__ uint16_t s_bef[256 * 12];
+    __shared__ __align__(16) uint16_t s_bef[256 * 12];
     __shared__ uint16_t s_used[256];
     for (int i = threadIdx.x; i < 256; i += blockDim.x) {
         s_tri[i] = tab->tri[i];
         s_used[i] = tab->used[i];
     }
-    for (int i = threadIdx.x; i < 256 * 12; i += blockDim.x) s_bef[i] = (&tab->before[0][0])[i];
+    uw_lds_load_before(s_bef, tab, int(threadIdx.x), int(blockDim.x));
     __syncthreads();
     for (uint32_t a = threadIdx.x; a < ci.n_active; a += blockDim.x) {
         const uint32_t ent = act[cbase + a];

@@ -19,7 +19,7 @@ __global__ void __launch_bounds__(kBlock) k_density_at(uw_field f, const uw_tabl
     for (int i = threadIdx.x; i < 256; i += kBlock) {
         const uint8_t p = tab->perm[i];
         P[i] = p;
-        const float4 g = tab->grad[p & 15];
+        const float4 g = uw_grad_vec(p & 15u);
         G3[3 * i] = g.x;
         G3[3 * i + 1] = g.y;
         G3[3 * i + 2] = g.z;

@@ -357,7 +357,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
     __shared__ uint32_t LIB[kTile * kTile];   // index position of each listed cell's first index entry
     __shared__ uint16_t T2C[kT2C];            // plane-local triangle -> list slot
     __shared__ uint32_t LUT[256];
-    __shared__ uint16_t BEF[256 * 12];
+    __shared__ __align__(16) uint16_t BEF[256 * 12];
     __shared__ uint32_t s_cnt;  // listed cells | triangles << 16
     const uw_totals tot = *a.totals;
     if (tot.overflow || tot.n_inds == 0) return;
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
     if (X0 >= X1) return;  // a slab past the grid's last plane (whole workgroup)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < 256; i += kVolBlock) LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(a.tab->tri[i] & 15u) << 16);
-    for (int i = tid; i < 256 * 12; i += kVolBlock) BEF[i] = (&a.tab->before[0][0])[i];
+    uw_lds_load_before(BEF, a.tab, tid, kVolBlock);
     const int32_t xs = X0 > 0 ? X0 - 1 : X0, ys = Y0 > 0 ? Y0 - 1 : Y0, zs = Z0 > 0 ? Z0 - 1 : Z0;
     const int32_t ny = Y1 - ys, nz = Z1 - zs;  // cells per plane row / column, halo included
     const int32_t sy = ny + 1, sz = nz + 1;
@@ -554,7 +554,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_backward(VolArgs a, const uw_
     for (int i = tid; i < 256; i += kVolBlock) {
         const uint8_t p = a.tab->perm[i];
         P[i] = p;
-        const float4 g = a.tab->grad[p & 15];
+        const float4 g = uw_grad_vec(p & 15u);
         G3[3 * i] = g.x;
         G3[3 * i + 1] = g.y;
         G3[3 * i + 2] = g.z;
