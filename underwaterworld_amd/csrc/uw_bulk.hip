@@ -339,27 +339,26 @@ __global__ void __launch_bounds__(kBlock) k_bulk_scan(BulkArgs a) {
 // ---------------------------------------------------------------------------------------------
 // Emission of one x-slab of a chunk: cell planes xs = X0 - 1 (when X0 > 0: classified for owner
 // lookups only) to X1 - 1. The slab's sample planes and row bases are fetched at once (one memory
-// round trip per workgroup). Per emitted plane the active cells are compacted into a list, then one
-// lane per listed cell writes its vertices and one lane per triangle its three index entries.
-constexpr int kT2C = 2048;  // triangles of a plane taken one per lane (more: one lane per cell)
+// round trip per workgroup). Classifying an emitted plane also writes its vertex map (one entry per
+// owned vertex record: cell, edge, chunk-local id) and triangle map (one entry per triangle: cell,
+// triangle of the cell, row-local triangle prefix), so the emission that follows takes one vertex,
+// then one triangle, per lane: no lane idles on a cell with fewer records than its neighbours'.
+// A plane with more records or triangles than a map holds falls back to one lane per cell.
+constexpr int kMapCap = 1280;  // entries of each per-plane map (LDS)
 
-// P: cell planes emitted per workgroup (the host takes fewer for launches of few chunks, whose
-// workgroups would otherwise not fill the device).
 template <int P>
 __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
     constexpr int kEmitPlanes = P;
     __shared__ float SP[2][kMaxL * kMaxL];     // sample planes x, x + 1 ([z][y], pitch L)
     __shared__ uint32_t CB[2][kMaxN * kMaxN];  // chunk-local first vertex id: cells of planes x - 1, x
     __shared__ uint8_t CC[2][kMaxN * kMaxN];   // cube cases of the same cells
-    // active cells of plane x: y 32 + z | plane-local triangle prefix << 10 | row-local triangle
-    // prefix << 23 (the cell's index entries start at 3 (row's triangle base + row-local prefix))
-    __shared__ uint32_t LIST[kMaxN * kMaxN];
-    __shared__ uint16_t T2C[kT2C];             // plane-local triangle -> list slot
+    __shared__ uint32_t V2C[kMapCap];          // cell y 32 + z | edge << 10 | chunk-local vertex id << 14
+    __shared__ uint32_t T2C[kMapCap];          // cell y 32 + z | triangle << 10 | row-local triangle prefix << 13
     __shared__ uint64_t TRI[256];
     __shared__ uint32_t LUT[256];
     __shared__ __align__(16) uint16_t BEF[256 * 12];
     __shared__ unsigned long long RB[(kEmitPlanes + 1) * kMaxN];  // row bases of cell planes xs .. X1 - 1
-    __shared__ uint32_t s_cnt;  // listed cells | triangles << 16
+    __shared__ unsigned long long s_cnt;  // vertex records | triangles << 32 of the emitted plane
     BulkSlab s;
     if (!bulk_slab(a, kEmitPlanes, &s)) return;
     const unsigned long long* rec = a.rec + 4 * uint64_t(s.j);
@@ -383,7 +382,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
         LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(row & 15u) << 16);
     }
     uw_lds_load_before(BEF, a.tab, tid, kBlock);
-    if (tid == 0) s_cnt = 0u;
+    if (tid == 0) s_cnt = 0ull;
     plane_put(SP[0], LL, pr[0]);
     plane_put(SP[1], LL, pr[1]);
     __syncthreads();
@@ -392,14 +391,15 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
     const bool coll = eo.tri_normals || eo.tri_cells;
     const float step = float(a.f.extent) / float(n);
     const float ox = float(a.f.extent * s.cx), oy = float(a.f.extent * s.cy), oz = float(a.f.extent * s.cz);
-#pragma unroll
+    static_assert(kEmitPlanes <= 4, "the plane hand-over below covers slabs of up to 4 planes");
+#pragma unroll 1
     for (int pl = 0; pl <= kEmitPlanes; ++pl) {
         const int32_t x = xs + pl;
         if (x >= s.X1) break;  // block-uniform
         const float* p0 = SP[pl & 1];
         const float* p1 = SP[(pl + 1) & 1];
         const bool emit_plane = x >= s.X0;
-        // classification of plane x and, when it is emitted, the list of its active cells
+        // classification of plane x; for an emitted plane, its vertex and triangle maps
         for (int32_t q = wave; 2 * q < n; q += kBlock / 64) {
             const int32_t y = 2 * q + (lane >> 5), z = lane & 31;
             const bool in = y < n && z < n;
@@ -409,34 +409,72 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
                 plane_corners(p0, p1, L, y, z, r);
                 cs = uw_case_of(r, iso);
                 const uint32_t w = LUT[cs];
-                const uint32_t nt = w >> 16;
-                // records (< 2^10), triangles (< 2^10) and active cells (< 2^6) of a row of 32
-                v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | (nt << 10) | ((nt ? 1u : 0u) << 20);
+                // records (< 2^10) and triangles of a row of 32
+                v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | ((w >> 16) << 10);
             }
             const uint32_t inc = half_incl_scan(v);
-            uint32_t old = 0;
-            if (emit_plane && (lane & 31) == 31 && y < n && (inc >> 20))
-                old = atomicAdd(&s_cnt, (inc >> 20) | (((inc >> 10) & 0x3FFu) << 16));
-            const uint32_t lbase = (lane & 32) ? uint32_t(__builtin_amdgcn_readlane(int(old), 63))
-                                               : uint32_t(__builtin_amdgcn_readlane(int(old), 31));
+            unsigned long long old = 0;
+            if (emit_plane && (lane & 31) == 31 && y < n && inc)
+                old = atomicAdd(&s_cnt, (unsigned long long)(inc & 0x3FFu) | ((unsigned long long)(inc >> 10) << 32));
+            const int src_lane = (lane & 32) ? 63 : 31;
+            const uint32_t vb = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old)), 31));
+            const uint32_t vb_hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old)), 63));
+            const uint32_t tb = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old >> 32)), 31));
+            const uint32_t tb_hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old >> 32)), 63));
+            const uint32_t vbase = src_lane == 63 ? vb_hi : vb, tbase = src_lane == 63 ? tb_hi : tb;
             if (in) {
                 const unsigned long long rb = RB[pl * kMaxN + y];
                 const uint32_t ex = inc - v;
                 const int o = y * kMaxN + z;
-                CB[x & 1][o] = uint32_t(rb) + (ex & 0x3FFu);
+                const uint32_t myv = uint32_t(rb) + (ex & 0x3FFu);
+                CB[x & 1][o] = myv;
                 CC[x & 1][o] = uint8_t(cs);
-                if (emit_plane && (v >> 20)) {
-                    const uint32_t sl = (lbase & 0xFFFFu) + (ex >> 20);
-                    LIST[sl] = uint32_t(o) | (((lbase >> 16) + ((ex >> 10) & 0x3FFu)) << 10) |
-                               (((ex >> 10) & 0xFFu) << 23);
+                if (emit_plane && (v >> 10)) {
+                    const uint32_t owned = uw_owned_mask(x, y, z);
+                    const uint32_t vp = vbase + (ex & 0x3FFu);
+                    for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
+                        const uint32_t ed = uint32_t(__builtin_ctz(vm));
+                        const uint32_t rank = __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
+                        if (vp + rank < uint32_t(kMapCap)) V2C[vp + rank] = uint32_t(o) | (ed << 10) | ((myv + rank) << 14);
+                    }
+                    const uint32_t tp = tbase + (ex >> 10), ntri = v >> 10;
+                    for (uint32_t i = 0; i < ntri; ++i)
+                        if (tp + i < uint32_t(kMapCap)) T2C[tp + i] = uint32_t(o) | (i << 10) | ((ex >> 10) << 13);
                 }
             }
         }
         __syncthreads();
         if (emit_plane) {
-            const uint32_t cnt = s_cnt;
-            const uint32_t na = cnt & 0xFFFFu, ntp = cnt >> 16;
-            const bool flat = ntp <= uint32_t(kT2C);  // block-uniform
+            const unsigned long long cnt = s_cnt;
+            const uint32_t nvp = uint32_t(cnt), ntp = uint32_t(cnt >> 32);
+            // one vertex record at its chunk-local id: corner densities of the edge's two ends
+            auto put_vertex = [&](int32_t y, int32_t z, uint32_t ed, uint32_t idl, float ra, float rb) {
+                const uint64_t id = vo + idl;
+                if (id >= vlim) return;
+                float p[3];
+                uw_edge_vertex_ab(ed, x, y, z, ra, rb, iso, step, ox, oy, oz, p);
+                float* dst = a.verts + eo.vstride * id;
+                dst[0] = p[0];
+                dst[1] = p[1];
+                dst[2] = p[2];
+                if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
+                if (a.tape) {
+                    const uint32_t ca = uw_corner_ofs(uw_edge_a(ed)), cbo = uw_corner_ofs(uw_edge_b(ed));
+                    uw_vertex_tape tv;
+                    tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L + z + int32_t((ca >> 2) & 1u));
+                    tv.b = uint32_t(((x + int32_t(cbo & 1u)) * L + y + int32_t((cbo >> 1) & 1u)) * L + z +
+                                    int32_t((cbo >> 2) & 1u));
+                    tv.rho_a = ra;
+                    tv.rho_b = rb;
+                    a.tape[id] = tv;
+                }
+            };
+            // density at corner c of cell (y, z) of plane x
+            auto corner = [&](int32_t y, int32_t z, uint32_t c) -> float {
+                const uint32_t d = uw_corner_ofs(c);
+                const float* pp = (d & 1u) ? p1 : p0;
+                return pp[(z + int32_t((d >> 2) & 1u)) * L + y + int32_t((d >> 1) & 1u)];
+            };
             // chunk-local id of the record that cell (x, y, z) sees as edge ed: its own rank, or its owner's
             auto edge_id = [&](int32_t y, int32_t z, uint32_t cs, uint32_t owned, uint32_t myv, uint32_t ed) -> uint32_t {
                 if (owned & (1u << ed)) return myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
@@ -446,16 +484,20 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
                 const int qo = qy * kMaxN + qz;
                 return CB[qx & 1][qo] + __popc(uint32_t(BEF[CC[qx & 1][qo] * 12 + qe]) & uw_owned_mask(qx, qy, qz));
             };
-            // index entries (and collision records) of triangle tj of a listed cell; gi: its first entry
-            auto emit_tri = [&](int32_t y, int32_t z, uint32_t cs, uint32_t owned, uint32_t myv, uint64_t row,
-                                uint32_t tj, uint64_t gi, const float* r) {
+            // index entries (and collision records) of triangle tj of an active cell; gi: its first entry
+            auto emit_tri = [&](int32_t y, int32_t z, uint32_t cs, uint32_t tj, uint64_t gi) {
+                const uint32_t owned = uw_owned_mask(x, y, z);
+                const uint32_t myv = CB[x & 1][y * kMaxN + z];
+                const uint64_t row = TRI[cs];
                 float q[3][3];
 #pragma unroll
                 for (uint32_t k = 0; k < 3; ++k) {
                     const uint32_t ed = uint32_t(row >> (4 + 4 * (3 * tj + k))) & 15u;
                     const uint32_t id = edge_id(y, z, cs, owned, myv, ed);
                     if (gi + k < ilim) uw_store_index(a.inds, eo.ind16, gi + k, id);
-                    if (coll) uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, q[k]);
+                    if (coll)
+                        uw_edge_vertex_ab(ed, x, y, z, corner(y, z, uw_edge_a(ed)), corner(y, z, uw_edge_b(ed)), iso, step,
+                                          ox, oy, oz, q[k]);
                 }
                 if (coll && gi + 2 < ilim) {
                     const uint64_t t = gi / 3;
@@ -469,72 +511,60 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
                     if (eo.tri_cells) eo.tri_cells[t] = uint32_t((x * n + y) * n + z);
                 }
             };
-            // first index entry of a listed cell: from its row's triangle base and row-local prefix
-            auto first_index = [&](uint32_t ent, int32_t y) -> uint64_t {
-                return io + 3u * (uint32_t(RB[pl * kMaxN + y] >> 32) + (ent >> 23));
-            };
-            for (uint32_t k = tid; k < na; k += kBlock) {
-                const uint32_t ent = LIST[k];
-                const int32_t y = int32_t((ent & 0x3FFu) >> 5), z = int32_t(ent & 31u);
-                const int o = y * kMaxN + z;
-                const uint32_t cs = CC[x & 1][o];
-                float r[8];
-                plane_corners(p0, p1, L, y, z, r);
-                const uint32_t owned = uw_owned_mask(x, y, z);
-                const uint32_t myv = CB[x & 1][o];
-                for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
-                    const uint32_t ed = uint32_t(__builtin_ctz(vm));
-                    const uint64_t id = vo + myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
-                    if (id >= vlim) continue;
-                    float p[3];
-                    uw_edge_vertex(ed, x, y, z, r, iso, step, ox, oy, oz, p);
-                    float* dst = a.verts + eo.vstride * id;
-                    dst[0] = p[0];
-                    dst[1] = p[1];
-                    dst[2] = p[2];
-                    if (eo.vstride == 6) uw_vertex_color(p[2], uw_edge_b(ed), float(a.f.extent), dst + 3);
-                    if (a.tape) {
-                        const uint32_t ia = uw_edge_a(ed), ib = uw_edge_b(ed);
-                        const uint32_t ca = uw_corner_ofs(ia), cbo = uw_corner_ofs(ib);
-                        uw_vertex_tape tv;
-                        tv.a = uint32_t(((x + int32_t(ca & 1u)) * L + y + int32_t((ca >> 1) & 1u)) * L + z +
-                                        int32_t((ca >> 2) & 1u));
-                        tv.b = uint32_t(((x + int32_t(cbo & 1u)) * L + y + int32_t((cbo >> 1) & 1u)) * L + z +
-                                        int32_t((cbo >> 2) & 1u));
-                        tv.rho_a = r[ia];
-                        tv.rho_b = r[ib];
-                        a.tape[id] = tv;
+            if (nvp <= uint32_t(kMapCap)) {
+                for (uint32_t t = tid; t < nvp; t += kBlock) {
+                    const uint32_t ent = V2C[t];
+                    const int32_t o = int32_t(ent & 0x3FFu), y = o >> 5, z = o & 31;
+                    const uint32_t ed = (ent >> 10) & 15u;
+                    put_vertex(y, z, ed, ent >> 14, corner(y, z, uw_edge_a(ed)), corner(y, z, uw_edge_b(ed)));
+                }
+            } else {  // one lane per cell
+                for (int32_t o = tid; o < n * kMaxN; o += kBlock) {
+                    const int32_t y = o >> 5, z = o & 31;
+                    if (z >= n) continue;
+                    const uint32_t cs = CC[x & 1][o];
+                    const uint32_t owned = uw_owned_mask(x, y, z), myv = CB[x & 1][o];
+                    for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
+                        const uint32_t ed = uint32_t(__builtin_ctz(vm));
+                        put_vertex(y, z, ed, myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned), corner(y, z, uw_edge_a(ed)),
+                                   corner(y, z, uw_edge_b(ed)));
                     }
                 }
-                const uint32_t ntri = LUT[cs] >> 16;
-                if (flat) {
-                    const uint32_t tp = (ent >> 10) & 0x1FFFu;
-                    for (uint32_t i = 0; i < ntri; ++i) T2C[tp + i] = uint16_t(k);
-                } else {
-                    const uint64_t row = TRI[cs];
-                    const uint64_t gi = first_index(ent, y);
-                    for (uint32_t tj = 0; tj < ntri; ++tj) emit_tri(y, z, cs, owned, myv, row, tj, gi + 3 * tj, r);
-                }
             }
-            if (flat) {
-                __syncthreads();  // the triangle map is complete
+            if (ntp <= uint32_t(kMapCap)) {
                 for (uint32_t t = tid; t < ntp; t += kBlock) {
-                    const uint32_t k = T2C[t];
-                    const uint32_t ent = LIST[k];
-                    const int32_t y = int32_t((ent & 0x3FFu) >> 5), z = int32_t(ent & 31u);
-                    const int o = y * kMaxN + z;
+                    const uint32_t ent = T2C[t];
+                    const int32_t o = int32_t(ent & 0x3FFu), y = o >> 5, z = o & 31;
+                    const uint32_t tj = (ent >> 10) & 7u;
+                    const uint64_t gi = io + 3u * (uint32_t(RB[pl * kMaxN + y] >> 32) + (ent >> 13) + tj);
+                    emit_tri(y, z, CC[x & 1][o], tj, gi);
+                }
+            } else {  // one lane per cell
+                for (int32_t o = tid; o < n * kMaxN; o += kBlock) {
+                    const int32_t y = o >> 5, z = o & 31;
+                    if (z >= n) continue;
                     const uint32_t cs = CC[x & 1][o];
-                    const uint32_t tj = t - ((ent >> 10) & 0x1FFFu);
-                    float r[8];
-                    if (coll) plane_corners(p0, p1, L, y, z, r);
-                    emit_tri(y, z, cs, uw_owned_mask(x, y, z), CB[x & 1][o], TRI[cs], tj, first_index(ent, y) + 3 * tj, r);
+                    const uint32_t ntri = LUT[cs] >> 16;
+                    if (!ntri) continue;
+                    // the cell's first triangle: the row's triangle base plus the triangles of the
+                    // row's cells before it
+                    uint32_t pre = 0;
+                    for (int32_t zz = 0; zz < z; ++zz) pre += LUT[CC[x & 1][y * kMaxN + zz]] >> 16;
+                    const uint64_t gi = io + 3u * (uint32_t(RB[pl * kMaxN + y] >> 32) + pre);
+                    for (uint32_t tj = 0; tj < ntri; ++tj) emit_tri(y, z, cs, tj, gi + 3 * tj);
                 }
             }
         }
         if (pl + 2 < kEmitPlanes + 2 && x + 1 < s.X1) {
-            __syncthreads();  // sample plane x and the plane's lists are read no more
-            if (tid == 0) s_cnt = 0u;
-            plane_put(SP[pl & 1], LL, pr[pl + 2]);  // plane x + 2
+            __syncthreads();  // sample plane x, the plane's maps and plane x - 1's cells are read no more
+            if (tid == 0) s_cnt = 0ull;
+            // plane x + 2 (a switch keeps the register array's indices static in the rolled loop)
+            switch (pl) {
+                case 0: plane_put(SP[0], LL, pr[2 < kEmitPlanes + 2 ? 2 : 0]); break;
+                case 1: plane_put(SP[1], LL, pr[3 < kEmitPlanes + 2 ? 3 : 0]); break;
+                case 2: plane_put(SP[0], LL, pr[4 < kEmitPlanes + 2 ? 4 : 0]); break;
+                default: plane_put(SP[1], LL, pr[5 < kEmitPlanes + 2 ? 5 : 0]); break;
+            }
             __syncthreads();
         }
     }

@@ -165,13 +165,12 @@ __device__ __forceinline__ void uw_store_index(uint32_t* inds, uint32_t ind16, u
     else inds[at] = id;
 }
 
-// World position of the lerp on edge e of cell (x, y, z) (chunk.rs:185-229), f32 with the
-// reference's operation order (the file is compiled with -ffp-contract=off).
-__device__ __forceinline__ void uw_edge_vertex(uint32_t e, int32_t x, int32_t y, int32_t z, const float r[8],
-                                               float iso, float step, float ox, float oy, float oz, float out[3]) {
-    const uint32_t ia = uw_edge_a(e), ib = uw_edge_b(e);
-    const uint32_t ca = uw_corner_ofs(ia), cb = uw_corner_ofs(ib);
-    const float ra = r[ia], rb = r[ib];
+// World position of the lerp on edge e of cell (x, y, z) (chunk.rs:185-229) from the densities ra,
+// rb of its start and end corners, f32 with the reference's operation order (the file is compiled
+// with -ffp-contract=off).
+__device__ __forceinline__ void uw_edge_vertex_ab(uint32_t e, int32_t x, int32_t y, int32_t z, float ra, float rb,
+                                                  float iso, float step, float ox, float oy, float oz, float out[3]) {
+    const uint32_t ca = uw_corner_ofs(uw_edge_a(e)), cb = uw_corner_ofs(uw_edge_b(e));
     const float t = (iso - ra) / (rb - ra);
     const float sax = float(x + int32_t(ca & 1u)) * step, sbx = float(x + int32_t(cb & 1u)) * step;
     const float say = float(y + int32_t((ca >> 1) & 1u)) * step, sby = float(y + int32_t((cb >> 1) & 1u)) * step;
@@ -179,4 +178,10 @@ __device__ __forceinline__ void uw_edge_vertex(uint32_t e, int32_t x, int32_t y,
     out[0] = (sax + t * (sbx - sax)) + ox;
     out[1] = (say + t * (sby - say)) + oy;
     out[2] = (saz + t * (sbz - saz)) + oz;
+}
+
+// The same from the cell's eight corner densities r (corner order of chunk.rs:144-153).
+__device__ __forceinline__ void uw_edge_vertex(uint32_t e, int32_t x, int32_t y, int32_t z, const float r[8],
+                                               float iso, float step, float ox, float oy, float oz, float out[3]) {
+    uw_edge_vertex_ab(e, x, y, z, r[uw_edge_a(e)], r[uw_edge_b(e)], iso, step, ox, oy, oz, out);
 }
