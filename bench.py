@@ -93,7 +93,8 @@ def main():
     import numpy as np
     import torch
 
-    from underwaterworld_amd import BatchedGraph, Perlin, _lib, get_batch_tuning, params, sample_density
+    from underwaterworld_amd import (BatchedGraph, Perlin, _lib, build_chunks_batched, get_batch_tuning, params,
+                                     sample_density)
     from underwaterworld_amd import build_chunks_plain, set_batch_tuning
     from underwaterworld_amd.engine import _alloc_out, _mesh_out, batch_capacity, tuned_shape
 
@@ -287,6 +288,12 @@ def main():
         g2 = BatchedGraph(pmix, mc, mr, vert_cap=mv, ind_cap=mi)
         ms_mixed = timed(lambda: g2.launch(stream.cuda_stream), max(3, args.steps // 4), 1)
         mt = g2.batch.totals()
+        # per-stage kernel times of the same build launched eagerly (a graph records no timers)
+        st_m = stage_ms(pmix, lambda: build_chunks_batched(pmix, mc, mr, vert_cap=mv, ind_cap=mi,
+                                                           stream=stream.cuda_stream), 3)
+        mixed_stages = {name: round(st_m[i], 4) for name, i in (
+            ("setup", _lib.STAGE_BATCH_SETUP), ("bulk_front", _lib.STAGE_BULK_FRONT),
+            ("persistent", _lib.STAGE_DOMINANT), ("bulk_emit", _lib.STAGE_BULK_EMIT)) if st_m[i] is not None}
         samples = int(sum((r + 1) ** 3 for r in mr))
         flops_m = flops_per_sample(pm["octaves"]) * samples
         res["mixed"] = {"workload": "8192 chunks, 2048 each at 8^3/12^3/24^3/32^3 cells, coords uniform in 32x32x8 "
@@ -295,7 +302,8 @@ def main():
                         "samples_per_s": round(samples / (ms_mixed / 1e3), 1), "samples": samples,
                         "algorithmic_tflops": round(flops_m / (ms_mixed / 1e3) / 1e12, 3),
                         "nonblank_chunks": int(mt["n_nonblank"]), "verts": int(mt["n_verts"]),
-                        "inds": int(mt["n_inds"])}
+                        "inds": int(mt["n_inds"]), "eager_stages_ms": mixed_stages,
+                        "bulk_min": get_batch_tuning(pmix)["bulk_min"]}
         log(f"# mixed workload: {ms_mixed:.4f} ms/step")
         g2.close()
         del g2
@@ -320,13 +328,19 @@ def main():
         ms_adj = timed(adj_step, max(3, args.steps // 4), 1)
         st_f = stage_ms(pad, lambda: mesher.forward(stream.cuda_stream), 3)
         st_a = stage_ms(pad, lambda: mesher.backward(gv, stream.cuda_stream, record=rec), 3)
+        # the forward's launches: queue set-up, the bulk path's front and emission (24^3 chunks leave
+        # the persistent queue when the tuning's bulk_min allows) and the persistent kernel
+        fwd_stages = {name: round(st_f[i], 4) for name, i in (
+            ("setup", _lib.STAGE_BATCH_SETUP), ("bulk_front", _lib.STAGE_BULK_FRONT),
+            ("persistent", _lib.STAGE_DOMINANT), ("bulk_emit", _lib.STAGE_BULK_EMIT)) if st_f[i] is not None}
+        fwd_ms = sum(fwd_stages.values())
         res["adjoint"] = {"workload": "256 chunks (x,y in [-4,4), z in [-3,1)) x 24^3 cells, 6 octaves, seed 42; "
                                       "forward with vertex tape + backward to 25^3 densities, 6 amplitudes, 3 offsets",
                           "ms_per_step": round(ms_adj, 4), "grads_per_s": round(len(ac) / (ms_adj / 1e3), 1),
-                          "forward_kernel_ms": round(st_f[_lib.STAGE_DOMINANT], 4),
+                          "forward_kernels_ms": round(fwd_ms, 4), "forward_stages_ms": fwd_stages,
                           "backward_kernel_ms": round(st_a[_lib.STAGE_DOMINANT], 4), "vertices": nvt,
                           "forward_algorithmic_tflops": round(
-                              flops_per_sample(pa["octaves"]) * 25 ** 3 * len(ac) / (st_f[0] / 1e3) / 1e12, 3)}
+                              flops_per_sample(pa["octaves"]) * 25 ** 3 * len(ac) / (fwd_ms / 1e3) / 1e12, 3)}
         log(f"# adjoint: {ms_adj:.4f} ms/step (forward + backward)")
         del mesher, rec, gv
 
