@@ -52,6 +52,7 @@ struct BulkArgs {
     int32_t packed;
     float* dens;                  // [j][Lm^3], chunk layout [x][z][y] with the chunk's own L
     unsigned long long* rows;     // [j][max_res^2]: per row (x n + y) records | triangles << 32
+    uint32_t* cases;              // [j][max_res][32 x 32 bytes]: cube case of cell (x, y, z) at x, y 32 + z
     uint32_t* below;              // [j] early-blank bits, zeroed by k_place
     uint32_t* act;                // [j] active cells, zeroed by k_place
     unsigned long long* rec;      // [j][4]: vertex base, index base, vertices, indices (~0: nothing to emit)
@@ -172,22 +173,28 @@ __global__ void __launch_bounds__(kBlock) k_bulk_density(BulkArgs a) {
 // planes; the next plane is fetched into registers while the current one is classified, so the
 // slab's plane loads overlap its work.
 constexpr int kCountPlanes = 8;  // cell planes per count workgroup
-constexpr int kPlaneRegs = (kMaxL * kMaxL + kBlock - 1) / kBlock;
+constexpr int kEmitBlock = 256;  // threads per emit workgroup (512 measured slower)
 
-struct PlaneRegs {
-    float v[kPlaneRegs];
+// One sample plane (up to kMaxL^2 floats) spread over the registers of a B-thread workgroup.
+template <int B>
+struct PlaneRegsT {
+    static constexpr int kRegs = (kMaxL * kMaxL + B - 1) / B;
+    float v[kRegs];
 };
-__device__ __forceinline__ void plane_fetch(const float* __restrict__ src, int32_t LL, PlaneRegs& r) {
+using PlaneRegs = PlaneRegsT<kBlock>;
+template <int B>
+__device__ __forceinline__ void plane_fetch(const float* __restrict__ src, int32_t LL, PlaneRegsT<B>& r) {
 #pragma unroll
-    for (int i = 0; i < kPlaneRegs; ++i) {
-        const int e = int(threadIdx.x) + i * kBlock;
+    for (int i = 0; i < PlaneRegsT<B>::kRegs; ++i) {
+        const int e = int(threadIdx.x) + i * B;
         r.v[i] = e < LL ? src[e] : 0.0f;
     }
 }
-__device__ __forceinline__ void plane_put(float* dst, int32_t LL, const PlaneRegs& r) {
+template <int B>
+__device__ __forceinline__ void plane_put(float* dst, int32_t LL, const PlaneRegsT<B>& r) {
 #pragma unroll
-    for (int i = 0; i < kPlaneRegs; ++i) {
-        const int e = int(threadIdx.x) + i * kBlock;
+    for (int i = 0; i < PlaneRegsT<B>::kRegs; ++i) {
+        const int e = int(threadIdx.x) + i * B;
         if (e < LL) dst[e] = r.v[i];
     }
 }
@@ -236,6 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_count(BulkArgs a) {
         if (x < s.X1) {
             const float* p0 = SP[p & 1];
             const float* p1 = SP[(p + 1) & 1];
+            uint8_t* cplane = reinterpret_cast<uint8_t*>(a.cases) + (uint64_t(s.j) * a.max_res + x) * (kMaxN * kMaxN);
             for (int32_t q = wave; 2 * q < n; q += kBlock / 64) {
                 const int32_t y = 2 * q + (lane >> 5), z = lane & 31;
                 uint32_t v = 0;
@@ -243,6 +251,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_count(BulkArgs a) {
                     float r[8];
                     plane_corners(p0, p1, L, y, z, r);
                     const uint32_t cs = uw_case_of(r, iso);
+                    cplane[y * kMaxN + z] = uint8_t(cs);  // for the emission, which then skips the corner loads
                     const uint32_t w = LUT[cs];
                     v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | ((w >> 16) << 10) |
                         ((cs != 0u && cs != 255u) ? (1u << 20) : 0u);
@@ -347,11 +356,11 @@ __global__ void __launch_bounds__(kBlock) k_bulk_scan(BulkArgs a) {
 constexpr int kMapCap = 1280;  // entries of each per-plane map (LDS)
 
 template <int P>
-__global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
+__global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
     constexpr int kEmitPlanes = P;
     __shared__ float SP[2][kMaxL * kMaxL];     // sample planes x, x + 1 ([z][y], pitch L)
     __shared__ uint32_t CB[2][kMaxN * kMaxN];  // chunk-local first vertex id: cells of planes x - 1, x
-    __shared__ uint8_t CC[2][kMaxN * kMaxN];   // cube cases of the same cells
+    __shared__ __align__(16) uint8_t CC[2][kMaxN * kMaxN];  // cube cases of the same cells (from k_bulk_count)
     __shared__ uint32_t V2C[kMapCap];          // cell y 32 + z | edge << 10 | chunk-local vertex id << 14
     __shared__ uint32_t T2C[kMapCap];          // cell y 32 + z | triangle << 10 | row-local triangle prefix << 13
     __shared__ uint64_t TRI[256];
@@ -370,21 +379,29 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
     const int32_t xs = s.X0 > 0 ? s.X0 - 1 : s.X0;
     const float* src = a.dens + uint64_t(s.j) * uint64_t(a.Lm) * a.Lm * a.Lm;
     const unsigned long long* rows = a.rows + uint64_t(s.j) * a.max_res * a.max_res;
-    PlaneRegs pr[kEmitPlanes + 2];  // sample planes xs .. X1
+    PlaneRegsT<kEmitBlock> pr[kEmitPlanes + 2];  // sample planes xs .. X1
 #pragma unroll
     for (int p = 0; p < kEmitPlanes + 2; ++p)
         if (xs + p <= s.X1) plane_fetch(src + int64_t(xs + p) * LL, LL, pr[p]);
+    // cube cases of cell planes xs .. X1 - 1, written by k_bulk_count (1 KB per plane, 4 per thread)
+    static_assert(kEmitBlock * 4 == kMaxN * kMaxN, "one case word per thread and plane");
+    const uint32_t* cw_src = a.cases + (uint64_t(s.j) * a.max_res) * (kMaxN * kMaxN / 4);
+    uint32_t cw[kEmitPlanes + 1];
+#pragma unroll
+    for (int p = 0; p <= kEmitPlanes; ++p)
+        cw[p] = xs + p < s.X1 ? cw_src[(xs + p) * (kMaxN * kMaxN / 4) + tid] : 0u;
     const int32_t nrb = (s.X1 - xs) * n;
     if (tid < nrb) RB[(tid / n) * kMaxN + tid % n] = rows[xs * n + tid];
-    for (int i = tid; i < 256; i += kBlock) {
+    for (int i = tid; i < 256; i += kEmitBlock) {
         const uint64_t row = a.tab->tri[i];
         TRI[i] = row;
         LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(row & 15u) << 16);
     }
-    uw_lds_load_before(BEF, a.tab, tid, kBlock);
+    uw_lds_load_before(BEF, a.tab, tid, kEmitBlock);
     if (tid == 0) s_cnt = 0ull;
     plane_put(SP[0], LL, pr[0]);
     plane_put(SP[1], LL, pr[1]);
+    reinterpret_cast<uint32_t*>(CC[xs & 1])[tid] = cw[0];
     __syncthreads();
     const float iso = a.f.iso;
     const uw_emit_opts& eo = a.eo;
@@ -400,14 +417,12 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
         const float* p1 = SP[(pl + 1) & 1];
         const bool emit_plane = x >= s.X0;
         // classification of plane x; for an emitted plane, its vertex and triangle maps
-        for (int32_t q = wave; 2 * q < n; q += kBlock / 64) {
+        for (int32_t q = wave; 2 * q < n; q += kEmitBlock / 64) {
             const int32_t y = 2 * q + (lane >> 5), z = lane & 31;
             const bool in = y < n && z < n;
             uint32_t v = 0, cs = 0;
             if (in) {
-                float r[8];
-                plane_corners(p0, p1, L, y, z, r);
-                cs = uw_case_of(r, iso);
+                cs = CC[x & 1][y * kMaxN + z];
                 const uint32_t w = LUT[cs];
                 // records (< 2^10) and triangles of a row of 32
                 v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | ((w >> 16) << 10);
@@ -428,7 +443,6 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
                 const int o = y * kMaxN + z;
                 const uint32_t myv = uint32_t(rb) + (ex & 0x3FFu);
                 CB[x & 1][o] = myv;
-                CC[x & 1][o] = uint8_t(cs);
                 if (emit_plane && (v >> 10)) {
                     const uint32_t owned = uw_owned_mask(x, y, z);
                     const uint32_t vp = vbase + (ex & 0x3FFu);
@@ -512,14 +526,14 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
                 }
             };
             if (nvp <= uint32_t(kMapCap)) {
-                for (uint32_t t = tid; t < nvp; t += kBlock) {
+                for (uint32_t t = tid; t < nvp; t += kEmitBlock) {
                     const uint32_t ent = V2C[t];
                     const int32_t o = int32_t(ent & 0x3FFu), y = o >> 5, z = o & 31;
                     const uint32_t ed = (ent >> 10) & 15u;
                     put_vertex(y, z, ed, ent >> 14, corner(y, z, uw_edge_a(ed)), corner(y, z, uw_edge_b(ed)));
                 }
             } else {  // one lane per cell
-                for (int32_t o = tid; o < n * kMaxN; o += kBlock) {
+                for (int32_t o = tid; o < n * kMaxN; o += kEmitBlock) {
                     const int32_t y = o >> 5, z = o & 31;
                     if (z >= n) continue;
                     const uint32_t cs = CC[x & 1][o];
@@ -532,7 +546,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
                 }
             }
             if (ntp <= uint32_t(kMapCap)) {
-                for (uint32_t t = tid; t < ntp; t += kBlock) {
+                for (uint32_t t = tid; t < ntp; t += kEmitBlock) {
                     const uint32_t ent = T2C[t];
                     const int32_t o = int32_t(ent & 0x3FFu), y = o >> 5, z = o & 31;
                     const uint32_t tj = (ent >> 10) & 7u;
@@ -540,7 +554,7 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
                     emit_tri(y, z, CC[x & 1][o], tj, gi);
                 }
             } else {  // one lane per cell
-                for (int32_t o = tid; o < n * kMaxN; o += kBlock) {
+                for (int32_t o = tid; o < n * kMaxN; o += kEmitBlock) {
                     const int32_t y = o >> 5, z = o & 31;
                     if (z >= n) continue;
                     const uint32_t cs = CC[x & 1][o];
@@ -558,12 +572,14 @@ __global__ void __launch_bounds__(kBlock) k_bulk_emit(BulkArgs a) {
         if (pl + 2 < kEmitPlanes + 2 && x + 1 < s.X1) {
             __syncthreads();  // sample plane x, the plane's maps and plane x - 1's cells are read no more
             if (tid == 0) s_cnt = 0ull;
-            // plane x + 2 (a switch keeps the register array's indices static in the rolled loop)
+            // sample plane x + 2 and the cases of cell plane x + 1 (whose slot held plane x - 1's, read
+            // no more); a switch keeps the register arrays' indices static in the rolled loop
+            uint32_t* cnext = reinterpret_cast<uint32_t*>(CC[(x + 1) & 1]);
             switch (pl) {
-                case 0: plane_put(SP[0], LL, pr[2 < kEmitPlanes + 2 ? 2 : 0]); break;
-                case 1: plane_put(SP[1], LL, pr[3 < kEmitPlanes + 2 ? 3 : 0]); break;
-                case 2: plane_put(SP[0], LL, pr[4 < kEmitPlanes + 2 ? 4 : 0]); break;
-                default: plane_put(SP[1], LL, pr[5 < kEmitPlanes + 2 ? 5 : 0]); break;
+                case 0: plane_put(SP[0], LL, pr[2 < kEmitPlanes + 2 ? 2 : 0]); cnext[tid] = cw[1 <= kEmitPlanes ? 1 : 0]; break;
+                case 1: plane_put(SP[1], LL, pr[3 < kEmitPlanes + 2 ? 3 : 0]); cnext[tid] = cw[2 <= kEmitPlanes ? 2 : 0]; break;
+                case 2: plane_put(SP[0], LL, pr[4 < kEmitPlanes + 2 ? 4 : 0]); cnext[tid] = cw[3 <= kEmitPlanes ? 3 : 0]; break;
+                default: plane_put(SP[1], LL, pr[5 < kEmitPlanes + 2 ? 5 : 0]); cnext[tid] = cw[4 <= kEmitPlanes ? 4 : 0]; break;
             }
             __syncthreads();
         }
@@ -583,9 +599,9 @@ BulkDensityKernel pick_bulk_density(int octaves) {
 
 size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
-int grid_for(uw_ctx* ctx, const void* k, size_t lds, int* grid) {
+int grid_for(uw_ctx* ctx, const void* k, size_t lds, int* grid, int block = kBlock) {
     int per_cu = 0;
-    UW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kBlock, lds));
+    UW_HIP(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, block, lds));
     if (per_cu < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "bulk kernel does not fit on a CU");
     *grid = per_cu * ctx->num_cus;
     return UW_OK;
@@ -615,7 +631,7 @@ int uw_bulk_shape_for(uw_ctx* ctx, const uw_batch_tuning& t, int32_t nchunks, uw
     // about four times over; a launch of few chunks gets shallow slabs, so more workgroups share
     // its chunks
     int resident = 0;
-    if (int rc = grid_for(ctx, reinterpret_cast<const void*>(k_bulk_emit<4>), 0, &resident)) return rc;
+    if (int rc = grid_for(ctx, reinterpret_cast<const void*>(k_bulk_emit<4>), 0, &resident, kEmitBlock)) return rc;
     s->emit_planes = 4;
     while (s->emit_planes > 1 && nc64 * ((t.max_res + s->emit_planes - 1) / s->emit_planes) < 4 * int64_t(resident))
         s->emit_planes /= 2;
@@ -631,6 +647,8 @@ int uw_bulk_shape_for(uw_ctx* ctx, const uw_batch_tuning& t, int32_t nchunks, uw
     o = align256(o + nc * size_t(Lm) * Lm * Lm * sizeof(float));
     s->rows_off = o;
     o = align256(o + nc * size_t(t.max_res) * t.max_res * sizeof(unsigned long long));
+    s->cases_off = o;
+    o = align256(o + nc * size_t(t.max_res) * kMaxN * kMaxN);
     s->below_off = o;
     o = align256(o + nc * sizeof(uint32_t));
     s->act_off = o;
@@ -658,6 +676,7 @@ BulkArgs bulk_args(uw_ctx* ctx, const uw_bulk_shape& s, const uw_bulk_io& io) {
     a.packed = io.packed;
     a.dens = reinterpret_cast<float*>(io.scratch + s.dens_off);
     a.rows = reinterpret_cast<unsigned long long*>(io.scratch + s.rows_off);
+    a.cases = reinterpret_cast<uint32_t*>(io.scratch + s.cases_off);
     a.below = reinterpret_cast<uint32_t*>(io.scratch + s.below_off);
     a.act = reinterpret_cast<uint32_t*>(io.scratch + s.act_off);
     a.rec = reinterpret_cast<unsigned long long*>(io.scratch + s.rec_off);
@@ -691,9 +710,9 @@ int uw_bulk_launch_back(uw_ctx* ctx, const uw_bulk_shape& s, const uw_bulk_io& i
     const BulkArgs a = bulk_args(ctx, s, io);
     if (int rc = uw_prof_start(ctx, st, UW_STAGE_BULK_EMIT)) return rc;
     switch (s.emit_planes) {
-        case 1: hipLaunchKernelGGL(k_bulk_emit<1>, dim3(uint32_t(s.grid_emit)), dim3(kBlock), 0, st, a); break;
-        case 2: hipLaunchKernelGGL(k_bulk_emit<2>, dim3(uint32_t(s.grid_emit)), dim3(kBlock), 0, st, a); break;
-        default: hipLaunchKernelGGL(k_bulk_emit<4>, dim3(uint32_t(s.grid_emit)), dim3(kBlock), 0, st, a); break;
+        case 1: hipLaunchKernelGGL(k_bulk_emit<1>, dim3(uint32_t(s.grid_emit)), dim3(kEmitBlock), 0, st, a); break;
+        case 2: hipLaunchKernelGGL(k_bulk_emit<2>, dim3(uint32_t(s.grid_emit)), dim3(kEmitBlock), 0, st, a); break;
+        default: hipLaunchKernelGGL(k_bulk_emit<4>, dim3(uint32_t(s.grid_emit)), dim3(kEmitBlock), 0, st, a); break;
     }
     if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());

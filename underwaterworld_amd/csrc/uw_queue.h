@@ -59,7 +59,7 @@ struct uw_bulk_shape {
     size_t lds_density;
     int grid_density, grid_count, grid_scan, grid_emit;
     int emit_planes;  // cell planes per emit workgroup: 1, 2 or 4
-    size_t dens_off, rows_off, below_off, act_off, rec_off, bytes;
+    size_t dens_off, rows_off, cases_off, below_off, act_off, rec_off, bytes;
 };
 
 // Buffers of one batched launch that the bulk kernels read or write.
