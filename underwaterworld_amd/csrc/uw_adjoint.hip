@@ -37,18 +37,71 @@ struct BackArgs {
     int64_t dens_stride;
     double* grad_amp;
     double* grad_off;
+    int32_t nchunks;
+    const void* tabs;  // per-chunk tables (BackTables<OMAX>), written by k_backward_tables
     int32_t nsub;   // workgroups per chunk, each taking every nsub-th run of kBlock vertices
     double* part;   // nsub > 1: per (chunk, share) parameter sums, [c][sub][OMAX + 3], added by k_backward_sum
 };
+
+// Per-chunk lattice data of the backward: every (axis, octave, sample) entry of the chunk (x hashed)
+// and the fade derivatives s'(d) = 30 d^2 (d - 1)^2, each at [k L + i] for the chunk's own L. Built once per launch by k_backward_tables (one thread per entry, so the f64 chains
+// of all chunks run side by side) and copied into LDS by each of the chunk's workgroups.
+template <int OMAX>
+struct BackTables {
+    uw_axis AX[OMAX * kMaxL], AY[OMAX * kMaxL], AZ[OMAX * kMaxL];
+    float DF[3][OMAX * kMaxL];
+};
+
+__device__ __forceinline__ bool back_skip(const BackArgs& a, int32_t c, int32_t* n) {
+    *n = a.res[c];
+    const uw_chunk_info ci = a.info[c];
+    return *n < 1 || *n > UW_BATCH_MAX_RESOLUTION || (ci.flags & (UW_CHUNK_OVERFLOW | UW_CHUNK_INVALID)) ||
+           ci.n_verts == 0;
+}
+
+template <int OMAX>
+__global__ void __launch_bounds__(256) k_backward_tables(BackArgs a, BackTables<OMAX>* __restrict__ out) {
+    constexpr int kAx = 3 * OMAX * kMaxL;  // axis entries: [axis][octave][sample]
+    constexpr int kE = 2 * kAx;            // + fade derivatives
+    const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int32_t c = int32_t(g / kE);
+    if (c >= a.nchunks) return;
+    int32_t n;
+    if (back_skip(a, c, &n)) return;
+    const int e = int(g - int64_t(c) * kE);
+    const int32_t L = n + 1, O = a.f.octaves;
+    const float step = float(a.f.extent) / float(n);
+    const int32_t cc[3] = {a.coords[3 * c], a.coords[3 * c + 1], a.coords[3 * c + 2]};
+    BackTables<OMAX>& t = out[c];
+    {
+        const int part = e / kAx, r0 = e - part * kAx;
+        const int axis = r0 / (OMAX * kMaxL), r = r0 - axis * (OMAX * kMaxL);
+        const int k = r / kMaxL, i = r - k * kMaxL;
+        if (k >= O || i >= L) return;
+        const double p = uw_sample_coord(a.f, axis, cc[axis], i, step);
+        if (part == 0) {
+            const uw_axis v = uw_axis_of(p, a.f.octave_freq[k]);
+            if (axis == 0) t.AX[k * L + i] = uw_hash_x(v, a.tab->perm);
+            else if (axis == 1) t.AY[k * L + i] = v;
+            else t.AZ[k * L + i] = v;
+        } else {
+            const double pk = p * a.f.octave_freq[k];
+            const double d = pk - floor(pk);
+            t.DF[axis][k * L + i] = float(30.0 * d * d * (d - 1.0) * (d - 1.0));
+        }
+    }
+}
 
 template <int OMAX>
 __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     __shared__ float G3[3 * 256];
     __shared__ uint8_t P[256];
-    __shared__ uw_axis AX[OMAX * kMaxL], AY[OMAX * kMaxL], AZ[OMAX * kMaxL];
-    __shared__ float DF[3][OMAX * kMaxL];
-    __shared__ float ADJ[kMaxL], POST[kMaxL];
+    __shared__ __align__(16) BackTables<OMAX> TB;
     __shared__ double red[kBlock / 64][OMAX + 3];
+    uw_axis* AX = TB.AX;
+    uw_axis* AY = TB.AY;
+    uw_axis* AZ = TB.AZ;
+    float(*DF)[OMAX * kMaxL] = TB.DF;
 
     const int tid = threadIdx.x;
     const int32_t c = int32_t(blockIdx.x / uint32_t(a.nsub)), sub = int32_t(blockIdx.x % uint32_t(a.nsub));
@@ -69,7 +122,24 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     }
     const int32_t L = n + 1;
     const float step = float(a.f.extent) / float(n);
-    const int32_t cx = a.coords[3 * c], cy = a.coords[3 * c + 1], cz = a.coords[3 * c + 2];
+    if (want_params) {  // the chunk's tables: 16-byte loads, all of a thread's issued before its stores
+        static_assert(sizeof(BackTables<OMAX>) % 16 == 0, "tables are copied in 16-byte words");
+        constexpr int kWords = int(sizeof(BackTables<OMAX>) / 16);
+        const uint4* src = reinterpret_cast<const uint4*>(static_cast<const BackTables<OMAX>*>(a.tabs) + c);
+        uint4* dst = reinterpret_cast<uint4*>(&TB);
+        for (int q0 = 0; q0 < kWords; q0 += 4 * kBlock) {  // four loads in flight per thread
+            const int e0 = q0 + tid, e1 = e0 + kBlock, e2 = e1 + kBlock, e3 = e2 + kBlock;
+            uint4 v0 = make_uint4(0, 0, 0, 0), v1 = v0, v2 = v0, v3 = v0;
+            if (e0 < kWords) v0 = src[e0];
+            if (e1 < kWords) v1 = src[e1];
+            if (e2 < kWords) v2 = src[e2];
+            if (e3 < kWords) v3 = src[e3];
+            if (e0 < kWords) dst[e0] = v0;
+            if (e1 < kWords) dst[e1] = v1;
+            if (e2 < kWords) dst[e2] = v2;
+            if (e3 < kWords) dst[e3] = v3;
+        }
+    }
     for (int i = tid; i < 256; i += kBlock) {
         const uint8_t p = a.tab->perm[i];
         P[i] = p;
@@ -77,19 +147,6 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
         G3[3 * i] = g.x;
         G3[3 * i + 1] = g.y;
         G3[3 * i + 2] = g.z;
-    }
-    __syncthreads();
-    const uw_chunk_tables T = {AX, AY, AZ, ADJ, POST};
-    if (want_params) {
-        uw_build_chunk_tables(a.f, P, cx, cy, cz, L, step, T, tid, kBlock);
-        const int32_t cc[3] = {cx, cy, cz};
-        for (int e = tid; e < 3 * O * L; e += kBlock) {
-            const int axis = e / (O * L), r = e - axis * O * L;
-            const int k = r / L, i = r - k * L;
-            const double pk = uw_sample_coord(a.f, axis, cc[axis], i, step) * a.f.octave_freq[k];
-            const double d = pk - floor(pk);
-            DF[axis][r] = float(30.0 * d * d * (d - 1.0) * (d - 1.0));
-        }
     }
     __syncthreads();
 
@@ -209,6 +266,22 @@ int launch_backward(uw_ctx* ctx, BackArgs a, int32_t nchunks, hipStream_t st) {
     }
     const uint64_t blocks = uint64_t(nchunks) * uint64_t(nsub);
     if (blocks > 0x7FFFFFFFull) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for one launch");
+    a.nchunks = nchunks;
+    a.tabs = nullptr;
+    if (want_params) {
+        const size_t bytes = size_t(nchunks) * sizeof(BackTables<OMAX>);
+        if (bytes > ctx->adj_tabs_bytes) {
+            (void)hipFree(ctx->d_adj_tabs);
+            ctx->d_adj_tabs = nullptr;
+            ctx->adj_tabs_bytes = 0;
+            UW_HIP(ctx, hipMalloc(&ctx->d_adj_tabs, bytes));
+            ctx->adj_tabs_bytes = bytes;
+        }
+        a.tabs = ctx->d_adj_tabs;
+        const uint64_t entries = uint64_t(nchunks) * (6 * OMAX * kMaxL);
+        hipLaunchKernelGGL(k_backward_tables<OMAX>, dim3(uint32_t((entries + 255) / 256)), dim3(256), 0, st, a,
+                           static_cast<BackTables<OMAX>*>(ctx->d_adj_tabs));
+    }
     hipLaunchKernelGGL(k_backward<OMAX>, dim3(uint32_t(blocks)), dim3(kBlock), 0, st, a);
     if (nsub > 1 && want_params) {
         const int64_t work = int64_t(nchunks) * (OMAX + 3);

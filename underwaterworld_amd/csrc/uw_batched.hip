@@ -492,17 +492,26 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uw_tables* tab = a.tab;
-    for (int i = tid; i < 256; i += BLOCK) {
-        const uint8_t p = tab->perm[i];
-        P[i] = p;
-        const float4 g = uw_grad_vec(p & 15u);
-        G3[3 * i] = g.x;
-        G3[3 * i + 1] = g.y;
-        G3[3 * i + 2] = g.z;
-        TRI[i] = tab->tri[i];
-        LUT[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
+    const bool pf = tid == BLOCK - 1;
+    const uint32_t nbig = *uw_queue_bulk_count(a.head);
+    if (pf) stage_task(a, nbig + blockIdx.x, s_next);
+    __syncthreads();
+    // a workgroup whose first task is past the end of the queue (a launch whose chunks all went to
+    // the bulk path, or fewer tasks than workgroups) skips the table loads and goes to the exit
+    const int32_t c_first = __builtin_amdgcn_readfirstlane(s_next[0]);
+    if (c_first >= 0 && c_first < a.nchunks) {
+        for (int i = tid; i < 256; i += BLOCK) {
+            const uint8_t p = tab->perm[i];
+            P[i] = p;
+            const float4 g = uw_grad_vec(p & 15u);
+            G3[3 * i] = g.x;
+            G3[3 * i + 1] = g.y;
+            G3[3 * i + 2] = g.z;
+            TRI[i] = tab->tri[i];
+            LUT[i] = uint32_t(tab->used[i]) | (uint32_t(tab->tri[i] & 15u) << 16);
+        }
+        uw_lds_load_before(BEF, tab, tid, BLOCK);
     }
-    uw_lds_load_before(BEF, tab, tid, BLOCK);
     __shared__ double s_freq[UW_MAX_OCTAVES];
     if (tid < UW_MAX_OCTAVES) s_freq[tid] = a.f.octave_freq[tid];
     const float iso = a.f.iso;
@@ -516,8 +525,6 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     // overlap the other lanes' work instead of opening the next task. Near the end of the queue,
     // and for multi-slab tasks (long enough that the round trips hardly count), the next task is
     // claimed when the block is free, which keeps the end of the launch balanced.
-    const bool pf = tid == BLOCK - 1;
-    if (pf) stage_task(a, *uw_queue_bulk_count(a.head) + blockIdx.x, s_next);
     uint32_t nonblank = 0;  // thread 0: chunks of this workgroup with indices
     bool staged = true;  // block-uniform: s_next holds the next task
 
@@ -860,11 +867,15 @@ __global__ void __launch_bounds__(BLOCK) k_batched(BatchArgs a) {
     // Launch totals, written by the last workgroup to leave: one 64-bit exit atomic per workgroup
     // (done | non-blank chunks << 32) on its own line, spread over the end of the launch; the
     // reservation atomics of every workgroup returned before its exit atomic, so the range
-    // counters read here (atomically, at the same coherence point) are final.
-    if (tid == 0) {
+    // counters read here (atomically, at the same coherence point) are final. The workgroups that
+    // had no task (g >= nchunks - nbig: they all leave at once, and their atomics would queue on
+    // the one line) are counted by one atomic of the first of them.
+    const uint32_t first_idle = nbig < uint32_t(a.nchunks) ? uint32_t(a.nchunks) - nbig : 0u;
+    const uint32_t add = blockIdx.x < first_idle ? 1u : (blockIdx.x == first_idle ? gridDim.x - first_idle : 0u);
+    if (tid == 0 && add) {
         unsigned long long* exitc = a.resv + kExitOffset;
-        const unsigned long long old = atomicAdd(exitc, 1ull | (static_cast<unsigned long long>(nonblank) << 32));
-        if (uint32_t(old) == gridDim.x - 1u) {
+        const unsigned long long old = atomicAdd(exitc, add | (static_cast<unsigned long long>(nonblank) << 32));
+        if (uint32_t(old) + add == gridDim.x) {
             const unsigned long long r0 = atomicAdd(a.resv, 0ull);
             a.totals->n_verts = a.packed ? (r0 & 0xFFFFFFFFull) : r0;
             a.totals->n_inds = a.packed ? (r0 >> 32) : atomicAdd(a.resv + kResvIndsOffset, 0ull);

@@ -79,6 +79,8 @@ struct uw_ctx {
     size_t batch_bulk_bytes = 0;
     double* d_adj_part = nullptr;  // adjoint backward: per-(chunk, vertex share) parameter sums
     size_t adj_part_bytes = 0;
+    void* d_adj_tabs = nullptr;    // adjoint backward: per-chunk lattice tables
+    size_t adj_tabs_bytes = 0;
     unsigned char* d_vol = nullptr;  // single-volume path: tables, density field, row counts, partial sums
     size_t vol_bytes = 0;
     uint8_t perm[256];
