@@ -325,7 +325,7 @@ def main():
             r = mesher.forward(stream.cuda_stream)
             mesher.backward(gv, stream.cuda_stream, record=r)
 
-        ms_adj = timed(adj_step, max(3, args.steps // 4), 1)
+        ms_adj = timed(adj_step, max(10, args.steps // 2), 2)
         st_f = stage_ms(pad, lambda: mesher.forward(stream.cuda_stream), 3)
         st_a = stage_ms(pad, lambda: mesher.backward(gv, stream.cuda_stream, record=rec), 3)
         # the forward's launches: queue set-up, the bulk path's front and emission (24^3 chunks leave

@@ -52,6 +52,12 @@ class _FieldScope:
 
     def __enter__(self):
         self.saved = self.perlin.field()
+        # a part equal to the current field is left alone (a backward usually runs on the field of
+        # its forward, which is still the context's): no library call, nothing to restore
+        if self.amps is not None and np.array_equal(np.asarray(self.amps, np.float64), self.saved[0]):
+            self.amps = None
+        if self.off is not None and np.array_equal(np.asarray(self.off, np.float64), self.saved[1]):
+            self.off = None
         if self.amps is not None or self.off is not None:
             self.perlin.set_field(self.amps, self.off)
         return self

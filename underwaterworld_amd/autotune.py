@@ -122,10 +122,13 @@ def refine_slabs(perlin, coords, res, best_ms, best, steps, stream, ref, log=pri
 
 
 def refine_bulk(perlin, coords, res, best_ms, best, steps, stream, ref, log=print):
-    """The bulk path's threshold: off (0), or each distinct resolution of the workload above the
-    one-pass sizes (chunks of at least bulk_min cells per axis leave the persistent queue)."""
+    """The bulk path's threshold: off (0), or, for each distinct resolution r of the workload above
+    the one-pass sizes, one more than the largest resolution below r (chunks of at least bulk_min
+    cells per axis leave the persistent queue, whose LDS is then sized for bulk_min - 1: the
+    tightest threshold that moves r and above gives the persistent kernel its smallest shape)."""
     results = []
-    cands = [0] + sorted({int(n) for n in res if 13 <= int(n) <= int(best["max_res"])})
+    rs = sorted({int(n) for n in res})
+    cands = [0] + [max([q for q in rs if q < r], default=0) + 1 for r in rs if 13 <= r <= int(best["max_res"])]
     for b in cands:
         if b == int(best.get("bulk_min", -1)):
             continue

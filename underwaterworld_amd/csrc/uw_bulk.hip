@@ -33,6 +33,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxN = UW_BATCH_MAX_RESOLUTION;
+constexpr size_t kBulkScratchMax = size_t(8) << 30;  // 8 GiB: about 55,000 chunks at max_res 32
 constexpr int kMaxL = kMaxN + 1;
 // Distinct x values among a density workgroup's 256 consecutive lines t = x L + y of a chunk with
 // L samples per axis: at most (256 - 1) / L + 2.
@@ -656,6 +657,12 @@ int uw_bulk_shape_for(uw_ctx* ctx, const uw_batch_tuning& t, int32_t nchunks, uw
     s->rec_off = o;
     o = align256(o + nc * 4 * sizeof(unsigned long long));
     s->bytes = o;
+    // a launch whose worst case would need more scratch than this keeps every chunk on the
+    // persistent kernel (same output, no per-chunk density slots)
+    if (s->bytes > kBulkScratchMax) {
+        memset(s, 0, sizeof(*s));
+        return UW_OK;
+    }
     return UW_OK;
 }
 
