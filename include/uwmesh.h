@@ -300,12 +300,13 @@ int uw_build_chunks_batched_tape(uw_ctx* ctx, const int32_t* coords, const int32
  *   tape        device tape of the forward
  *   grad_verts  device float[3 V]: dL/d(vertex position), laid out like the forward's verts
  *   grad_dens   device float[nchunks * dens_stride] or NULL: dL/d(density) per chunk sample,
- *               accumulated (the caller zeroes it); dens_stride >= (n+1)^3 of every chunk
+ *               written (all nchunks * dens_stride entries are zeroed first, then the
+ *               vertices' shares added); dens_stride >= (n+1)^3 of every chunk
  *   grad_amp    device double[nchunks * octaves] or NULL: per-chunk dL/dA_k (includes the
  *               normalisation by the amplitude sum)
  *   grad_off    device double[nchunks * 3] or NULL: per-chunk dL/d(offset)
- * Summation order across threads is not fixed (atomics into grad_dens), so results can differ in
- * the last bits between runs. */
+ * grad_amp and grad_off are summed in a fixed order for a given launch shape; grad_dens is summed
+ * by atomics, so it can differ in the last bits between runs. */
 int uw_adjoint_backward(uw_ctx* ctx, const int32_t* coords, const int32_t* res, int32_t nchunks,
                         const uw_chunk_info* info, const uw_vertex_tape* tape, const float* grad_verts,
                         float* grad_dens, int64_t dens_stride, double* grad_amp, double* grad_off, void* stream);

@@ -165,7 +165,8 @@ class ChunkMesher:
                       torch.empty((max(self.nchunks, 1), 3), dtype=torch.float64, device=dev), None)
         gd_ptr, stride = None, 0
         if self.want_dens_grad:
-            g.grad_dens = torch.zeros((max(self.nchunks, 1), self.max_samples), dtype=torch.float32, device=dev)
+            # written by the backward (zeroed there first)
+            g.grad_dens = torch.empty((max(self.nchunks, 1), self.max_samples), dtype=torch.float32, device=dev)
             gd_ptr, stride = ctypes.c_void_p(g.grad_dens.data_ptr()), self.max_samples
         with _FieldScope(self.perlin, rec.amplitudes, rec.offset):
             rc = _lib.lib().uw_adjoint_backward(self.perlin.ctx, ctypes.c_void_p(self.coords.data_ptr()),

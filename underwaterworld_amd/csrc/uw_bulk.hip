@@ -45,6 +45,7 @@ struct BulkArgs {
     const uw_task* tasks;
     uint32_t* qhead;            // queue state: bulk entry count, range and exit counters
     int32_t max_res;            // largest resolution of the launch (sizes the per-chunk slots)
+    int32_t entries;            // chunks of the launch: count and emit workgroups per slab row
     int32_t Lm;                 // max_res + 1
     int32_t wpc;                // density workgroup items per chunk: ceil(Lm^2 / 256)
     int32_t nx_tab;             // x entries per octave in the density kernel's LDS (smallest bulk L)
@@ -204,14 +205,20 @@ struct BulkSlab {
     int32_t c, n, cx, cy, cz, X0, X1;
     uint32_t j;
 };
-// The workgroup's chunk entry and slab; false when it has nothing to do.
+// The workgroup's chunk entry and slab; false when it has nothing to do. Workgroups are laid out
+// slab-major: workgroup b takes slab b / entries of chunk entry b % entries. Bulk entries are a
+// prefix of the queue, sorted by resolution (largest first), so the live workgroups of each slab
+// row form one contiguous run, which the dispatcher spreads evenly over the XCDs (workgroup b goes
+// to XCD b % 8) and their CUs. In chunk-major order the slabs past a smaller chunk's last plane
+// left gaps at a fixed stride: 24^3 chunks in a launch sized for 32^3 put all their live emission
+// workgroups of half the slabs on four of the eight XCDs, and within those on half of the CUs.
 __device__ __forceinline__ bool bulk_slab(const BulkArgs& a, int32_t planes, BulkSlab* s) {
-    const uint32_t slabs = uint32_t((a.max_res + planes - 1) / planes);
-    s->j = blockIdx.x / slabs;
+    const uint32_t row = blockIdx.x / uint32_t(a.entries);
+    s->j = blockIdx.x - row * uint32_t(a.entries);
     if (s->j >= bulk_count(a)) return false;
     load_task(a, s->j, &s->c, &s->n, &s->cx, &s->cy, &s->cz);
     if (s->n < 1 || s->n > a.max_res) return false;
-    s->X0 = int32_t(blockIdx.x - s->j * slabs) * planes;
+    s->X0 = int32_t(row) * planes;
     s->X1 = min(s->n, s->X0 + planes);
     return s->X0 < s->X1;
 }
@@ -627,6 +634,7 @@ int uw_bulk_shape_for(uw_ctx* ctx, const uw_batch_tuning& t, int32_t nchunks, uw
     if (int rc = grid_for(ctx, reinterpret_cast<const void*>(k_bulk_scan), 0, &s->grid_scan)) return rc;
     // count and emit: one workgroup per (chunk entry, x-slab) of every chunk of the launch
     const int64_t nc64 = nchunks > 0 ? nchunks : 1;
+    s->entries = int32_t(nc64);
     const int64_t gc = nc64 * ((t.max_res + kCountPlanes - 1) / kCountPlanes);
     // emit: the deepest slab (fewest halo planes) whose workgroups still cover the resident slots
     // about four times over; a launch of few chunks gets shallow slabs, so more workgroups share
@@ -675,6 +683,7 @@ BulkArgs bulk_args(uw_ctx* ctx, const uw_bulk_shape& s, const uw_bulk_io& io) {
     a.tasks = io.tasks;
     a.qhead = io.qhead;
     a.max_res = s.max_res;
+    a.entries = s.entries;
     a.Lm = s.max_res + 1;
     a.wpc = s.wpc;
     a.nx_tab = s.nx_tab;

@@ -245,6 +245,7 @@ void uw_ctx_destroy(uw_ctx* ctx) {
     (void)hipFree(ctx->d_batch_bulk);
     (void)hipFree(ctx->d_vol);
     (void)hipFree(ctx->d_adj_part);
+    (void)hipFree(ctx->d_adj_plan);
     (void)hipFree(ctx->d_adj_tabs);
     for (hipEvent_t e : ctx->prof_events) (void)hipEventDestroy(e);
     delete ctx;

@@ -77,8 +77,10 @@ struct uw_ctx {
     size_t batch_dscr_bytes = 0;
     unsigned char* d_batch_bulk = nullptr;  // bulk path scratch of the eager batched path
     size_t batch_bulk_bytes = 0;
-    double* d_adj_part = nullptr;  // adjoint backward: per-(chunk, vertex share) parameter sums
+    double* d_adj_part = nullptr;  // adjoint backward: per-(workgroup, chunk) parameter sums
     size_t adj_part_bytes = 0;
+    uint32_t* d_adj_plan = nullptr;  // adjoint backward: piece plan (piece starts, workgroup first chunks)
+    size_t adj_plan_bytes = 0;
     void* d_adj_tabs = nullptr;    // adjoint backward: per-chunk lattice tables
     size_t adj_tabs_bytes = 0;
     unsigned char* d_vol = nullptr;  // single-volume path: tables, density field, row counts, partial sums

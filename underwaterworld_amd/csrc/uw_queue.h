@@ -54,6 +54,7 @@ __device__ __forceinline__ void uw_reserve_ranges(unsigned long long* resv, int 
 struct uw_bulk_shape {
     bool on;         // tuning.bulk_min within [1, max_res]
     int32_t max_res; // largest resolution of the launch
+    int32_t entries; // chunks of the launch (count and emit grids: entries x slabs, slab-major)
     int32_t wpc;     // density workgroup items per chunk
     int32_t nx_tab;  // x entries per octave in the density kernel's LDS
     size_t lds_density;
