@@ -5,14 +5,14 @@
 //
 //   k_bulk_density  one lane per z-line (lanes on consecutive lines t = x L + y of one chunk),
 //                   densities into a per-chunk HBM field [x][z][y], so each sample's store is a row
-//   k_bulk_count    one workgroup per (chunk, x-slab of 8 cell planes): cube cases and, per row
+//   k_bulk_count    one workgroup per (chunk, x-slab of 8 or 4 cell planes): cube cases and, per row
 //                   (x, y), the owned vertex records and triangles (chunk.rs:141-164 with the
 //                   ownership rule)
 //   k_bulk_scan     one workgroup per chunk: the chunk's row bases (first-seen order within the
 //                   chunk), its totals and record, and one reservation of its vertex and index
 //                   ranges on the launch's range counter (the persistent kernel's reservations use
 //                   the same counter, so ranges never overlap)
-//   k_bulk_emit     one workgroup per (chunk, x-slab of 4 cell planes): classifies the slab's planes
+//   k_bulk_emit     one workgroup per (chunk, x-slab of 1, 2 or 4 cell planes): classifies the slab's planes
 //                   and the one below it (owners of shared records lie at most one plane back),
 //                   then writes the planes' vertices, indices and, when requested, tape, colours
 //                   and collision records
@@ -174,7 +174,8 @@ __global__ void __launch_bounds__(kBlock) k_bulk_density(BulkArgs a) {
 // entries or the chunk's planes leave at once). Sample planes pass through a ring of two LDS
 // planes; the next plane is fetched into registers while the current one is classified, so the
 // slab's plane loads overlap its work.
-constexpr int kCountPlanes = 8;  // cell planes per count workgroup
+// Cell planes per count workgroup: 8, or 4 when the launch has few chunks (the host's choice in
+// uw_bulk_shape_for), so that short launches still spread over the device.
 constexpr int kEmitBlock = 256;  // threads per emit workgroup (512 measured slower)
 
 // One sample plane (up to kMaxL^2 floats) spread over the registers of a B-thread workgroup.
@@ -224,9 +225,11 @@ __device__ __forceinline__ bool bulk_slab(const BulkArgs& a, int32_t planes, Bul
 }
 
 // Per-row counts of one x-slab of a chunk: rows[j][x n + y] = owned records | triangles << 32. The
-// slab's kCountPlanes + 1 sample planes are fetched into registers at once (one memory round trip
+// slab's CP + 1 sample planes are fetched into registers at once (one memory round trip
 // per workgroup) and pass through a ring of two LDS planes.
+template <int CP>
 __global__ void __launch_bounds__(kBlock) k_bulk_count(BulkArgs a) {
+    constexpr int kCountPlanes = CP;
     __shared__ float SP[2][kMaxL * kMaxL];
     __shared__ uint32_t LUT[256];
     BulkSlab s;
@@ -357,25 +360,28 @@ __global__ void __launch_bounds__(kBlock) k_bulk_scan(BulkArgs a) {
 // Emission of one x-slab of a chunk: cell planes xs = X0 - 1 (when X0 > 0: classified for owner
 // lookups only) to X1 - 1. The slab's sample planes and row bases are fetched at once (one memory
 // round trip per workgroup). Classifying an emitted plane also writes its vertex map (one entry per
-// owned vertex record: cell, edge, chunk-local id) and triangle map (one entry per triangle: cell,
-// triangle of the cell, row-local triangle prefix), so the emission that follows takes one vertex,
-// then one triangle, per lane: no lane idles on a cell with fewer records than its neighbours'.
-// A plane with more records or triangles than a map holds falls back to one lane per cell.
+// owned vertex record: cell and edge, at the record's id less the plane's first id) and triangle
+// map (one entry per triangle: cell and triangle of the cell, at the triangle's number less the
+// plane's first), so the emission that follows takes one vertex, then one triangle, per lane: no
+// lane idles on a cell with fewer records than its neighbours'. Map positions come from the row
+// bases of the chunk's scan, so no counter is shared. A plane with more records or triangles than
+// a map holds falls back to one lane per cell.
 constexpr int kMapCap = 1280;  // entries of each per-plane map (LDS)
 
 template <int P>
 __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
     constexpr int kEmitPlanes = P;
     __shared__ float SP[2][kMaxL * kMaxL];     // sample planes x, x + 1 ([z][y], pitch L)
-    __shared__ uint32_t CB[2][kMaxN * kMaxN];  // chunk-local first vertex id: cells of planes x - 1, x
+    __shared__ uint16_t CB[2][kMaxN * kMaxN];  // first vertex id within its row: cells of planes x - 1, x
     __shared__ __align__(16) uint8_t CC[2][kMaxN * kMaxN];  // cube cases of the same cells (from k_bulk_count)
-    __shared__ uint32_t V2C[kMapCap];          // cell y 32 + z | edge << 10 | chunk-local vertex id << 14
-    __shared__ uint32_t T2C[kMapCap];          // cell y 32 + z | triangle << 10 | row-local triangle prefix << 13
+    __shared__ uint16_t V2C[kMapCap];          // cell y 32 + z | edge << 10
+    __shared__ uint16_t T2C[kMapCap];          // cell y 32 + z | triangle of the cell << 10
     __shared__ uint64_t TRI[256];
     __shared__ uint32_t LUT[256];
     __shared__ __align__(16) uint16_t BEF[256 * 12];
-    __shared__ unsigned long long RB[(kEmitPlanes + 1) * kMaxN];  // row bases of cell planes xs .. X1 - 1
-    __shared__ unsigned long long s_cnt;  // vertex records | triangles << 32 of the emitted plane
+    // row bases (vertex records | triangles << 32, chunk-local) of cell planes xs .. X1 - 1, then the
+    // first row base of plane X1 (the chunk's totals when X1 is its last plane)
+    __shared__ unsigned long long RB[(kEmitPlanes + 2) * kMaxN];
     BulkSlab s;
     if (!bulk_slab(a, kEmitPlanes, &s)) return;
     const unsigned long long* rec = a.rec + 4 * uint64_t(s.j);
@@ -398,15 +404,16 @@ __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
 #pragma unroll
     for (int p = 0; p <= kEmitPlanes; ++p)
         cw[p] = xs + p < s.X1 ? cw_src[(xs + p) * (kMaxN * kMaxN / 4) + tid] : 0u;
-    const int32_t nrb = (s.X1 - xs) * n;
+    const int32_t nrb = (s.X1 - xs) * n;  // at most 5 planes of 32 rows: the last thread is free
     if (tid < nrb) RB[(tid / n) * kMaxN + tid % n] = rows[xs * n + tid];
+    if (tid == kEmitBlock - 1)
+        RB[(s.X1 - xs) * kMaxN] = s.X1 < n ? rows[s.X1 * n] : (rec[2] | ((rec[3] / 3u) << 32));
     for (int i = tid; i < 256; i += kEmitBlock) {
         const uint64_t row = a.tab->tri[i];
         TRI[i] = row;
         LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(row & 15u) << 16);
     }
     uw_lds_load_before(BEF, a.tab, tid, kEmitBlock);
-    if (tid == 0) s_cnt = 0ull;
     plane_put(SP[0], LL, pr[0]);
     plane_put(SP[1], LL, pr[1]);
     reinterpret_cast<uint32_t*>(CC[xs & 1])[tid] = cw[0];
@@ -424,6 +431,7 @@ __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
         const float* p0 = SP[pl & 1];
         const float* p1 = SP[(pl + 1) & 1];
         const bool emit_plane = x >= s.X0;
+        const unsigned long long rb0 = RB[pl * kMaxN];  // the plane's first vertex id and triangle
         // classification of plane x; for an emitted plane, its vertex and triangle maps
         for (int32_t q = wave; 2 * q < n; q += kEmitBlock / 64) {
             const int32_t y = 2 * q + (lane >> 5), z = lane & 31;
@@ -436,39 +444,30 @@ __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
                 v = __popc(w & 0xFFFu & uw_owned_mask(x, y, z)) | ((w >> 16) << 10);
             }
             const uint32_t inc = half_incl_scan(v);
-            unsigned long long old = 0;
-            if (emit_plane && (lane & 31) == 31 && y < n && inc)
-                old = atomicAdd(&s_cnt, (unsigned long long)(inc & 0x3FFu) | ((unsigned long long)(inc >> 10) << 32));
-            const int src_lane = (lane & 32) ? 63 : 31;
-            const uint32_t vb = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old)), 31));
-            const uint32_t vb_hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old)), 63));
-            const uint32_t tb = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old >> 32)), 31));
-            const uint32_t tb_hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(old >> 32)), 63));
-            const uint32_t vbase = src_lane == 63 ? vb_hi : vb, tbase = src_lane == 63 ? tb_hi : tb;
             if (in) {
                 const unsigned long long rb = RB[pl * kMaxN + y];
                 const uint32_t ex = inc - v;
                 const int o = y * kMaxN + z;
-                const uint32_t myv = uint32_t(rb) + (ex & 0x3FFu);
-                CB[x & 1][o] = myv;
+                CB[x & 1][o] = uint16_t(ex & 0x3FFu);
                 if (emit_plane && (v >> 10)) {
                     const uint32_t owned = uw_owned_mask(x, y, z);
-                    const uint32_t vp = vbase + (ex & 0x3FFu);
+                    // map positions: the cell's first id and triangle, less the plane's first
+                    const uint32_t vp = uint32_t(rb) - uint32_t(rb0) + (ex & 0x3FFu);
                     for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
                         const uint32_t ed = uint32_t(__builtin_ctz(vm));
                         const uint32_t rank = __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
-                        if (vp + rank < uint32_t(kMapCap)) V2C[vp + rank] = uint32_t(o) | (ed << 10) | ((myv + rank) << 14);
+                        if (vp + rank < uint32_t(kMapCap)) V2C[vp + rank] = uint16_t(uint32_t(o) | (ed << 10));
                     }
-                    const uint32_t tp = tbase + (ex >> 10), ntri = v >> 10;
+                    const uint32_t tp = uint32_t(rb >> 32) - uint32_t(rb0 >> 32) + (ex >> 10), ntri = v >> 10;
                     for (uint32_t i = 0; i < ntri; ++i)
-                        if (tp + i < uint32_t(kMapCap)) T2C[tp + i] = uint32_t(o) | (i << 10) | ((ex >> 10) << 13);
+                        if (tp + i < uint32_t(kMapCap)) T2C[tp + i] = uint16_t(uint32_t(o) | (i << 10));
                 }
             }
         }
         __syncthreads();
         if (emit_plane) {
-            const unsigned long long cnt = s_cnt;
-            const uint32_t nvp = uint32_t(cnt), ntp = uint32_t(cnt >> 32);
+            const unsigned long long rb1 = RB[(pl + 1) * kMaxN];  // the next plane's first id and triangle
+            const uint32_t nvp = uint32_t(rb1) - uint32_t(rb0), ntp = uint32_t(rb1 >> 32) - uint32_t(rb0 >> 32);
             // one vertex record at its chunk-local id: corner densities of the edge's two ends
             auto put_vertex = [&](int32_t y, int32_t z, uint32_t ed, uint32_t idl, float ra, float rb) {
                 const uint64_t id = vo + idl;
@@ -504,12 +503,13 @@ __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
                 uint32_t qe;
                 uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
                 const int qo = qy * kMaxN + qz;
-                return CB[qx & 1][qo] + __popc(uint32_t(BEF[CC[qx & 1][qo] * 12 + qe]) & uw_owned_mask(qx, qy, qz));
+                return uint32_t(RB[(qx - xs) * kMaxN + qy]) + CB[qx & 1][qo] +
+                       __popc(uint32_t(BEF[CC[qx & 1][qo] * 12 + qe]) & uw_owned_mask(qx, qy, qz));
             };
             // index entries (and collision records) of triangle tj of an active cell; gi: its first entry
             auto emit_tri = [&](int32_t y, int32_t z, uint32_t cs, uint32_t tj, uint64_t gi) {
                 const uint32_t owned = uw_owned_mask(x, y, z);
-                const uint32_t myv = CB[x & 1][y * kMaxN + z];
+                const uint32_t myv = uint32_t(RB[pl * kMaxN + y]) + CB[x & 1][y * kMaxN + z];
                 const uint64_t row = TRI[cs];
                 float q[3][3];
 #pragma unroll
@@ -537,15 +537,15 @@ __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
                 for (uint32_t t = tid; t < nvp; t += kEmitBlock) {
                     const uint32_t ent = V2C[t];
                     const int32_t o = int32_t(ent & 0x3FFu), y = o >> 5, z = o & 31;
-                    const uint32_t ed = (ent >> 10) & 15u;
-                    put_vertex(y, z, ed, ent >> 14, corner(y, z, uw_edge_a(ed)), corner(y, z, uw_edge_b(ed)));
+                    const uint32_t ed = ent >> 10;
+                    put_vertex(y, z, ed, uint32_t(rb0) + t, corner(y, z, uw_edge_a(ed)), corner(y, z, uw_edge_b(ed)));
                 }
             } else {  // one lane per cell
                 for (int32_t o = tid; o < n * kMaxN; o += kEmitBlock) {
                     const int32_t y = o >> 5, z = o & 31;
                     if (z >= n) continue;
                     const uint32_t cs = CC[x & 1][o];
-                    const uint32_t owned = uw_owned_mask(x, y, z), myv = CB[x & 1][o];
+                    const uint32_t owned = uw_owned_mask(x, y, z), myv = uint32_t(RB[pl * kMaxN + y]) + CB[x & 1][o];
                     for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
                         const uint32_t ed = uint32_t(__builtin_ctz(vm));
                         put_vertex(y, z, ed, myv + __popc(uint32_t(BEF[cs * 12 + ed]) & owned), corner(y, z, uw_edge_a(ed)),
@@ -557,9 +557,8 @@ __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
                 for (uint32_t t = tid; t < ntp; t += kEmitBlock) {
                     const uint32_t ent = T2C[t];
                     const int32_t o = int32_t(ent & 0x3FFu), y = o >> 5, z = o & 31;
-                    const uint32_t tj = (ent >> 10) & 7u;
-                    const uint64_t gi = io + 3u * (uint32_t(RB[pl * kMaxN + y] >> 32) + (ent >> 13) + tj);
-                    emit_tri(y, z, CC[x & 1][o], tj, gi);
+                    const uint64_t gi = io + 3ull * (uint64_t(uint32_t(rb0 >> 32)) + t);
+                    emit_tri(y, z, CC[x & 1][o], ent >> 10, gi);
                 }
             } else {  // one lane per cell
                 for (int32_t o = tid; o < n * kMaxN; o += kEmitBlock) {
@@ -579,7 +578,6 @@ __global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
         }
         if (pl + 2 < kEmitPlanes + 2 && x + 1 < s.X1) {
             __syncthreads();  // sample plane x, the plane's maps and plane x - 1's cells are read no more
-            if (tid == 0) s_cnt = 0ull;
             // sample plane x + 2 and the cases of cell plane x + 1 (whose slot held plane x - 1's, read
             // no more); a switch keeps the register arrays' indices static in the rolled loop
             uint32_t* cnext = reinterpret_cast<uint32_t*>(CC[(x + 1) & 1]);
@@ -635,7 +633,10 @@ int uw_bulk_shape_for(uw_ctx* ctx, const uw_batch_tuning& t, int32_t nchunks, uw
     // count and emit: one workgroup per (chunk entry, x-slab) of every chunk of the launch
     const int64_t nc64 = nchunks > 0 ? nchunks : 1;
     s->entries = int32_t(nc64);
-    const int64_t gc = nc64 * ((t.max_res + kCountPlanes - 1) / kCountPlanes);
+    int resident_count = 0;
+    if (int rc = grid_for(ctx, reinterpret_cast<const void*>(k_bulk_count<8>), 0, &resident_count)) return rc;
+    s->count_planes = nc64 * ((t.max_res + 7) / 8) < 2 * int64_t(resident_count) ? 4 : 8;
+    const int64_t gc = nc64 * ((t.max_res + s->count_planes - 1) / s->count_planes);
     // emit: the deepest slab (fewest halo planes) whose workgroups still cover the resident slots
     // about four times over; a launch of few chunks gets shallow slabs, so more workgroups share
     // its chunks
@@ -714,7 +715,10 @@ int uw_bulk_launch_front(uw_ctx* ctx, const uw_bulk_shape& s, const uw_bulk_io& 
     const BulkDensityKernel kd = pick_bulk_density(ctx->params.octaves);
     if (int rc = uw_prof_start(ctx, st, UW_STAGE_BULK_FRONT)) return rc;
     hipLaunchKernelGGL(kd, dim3(uint32_t(s.grid_density)), dim3(kBlock), s.lds_density, st, a);
-    hipLaunchKernelGGL(k_bulk_count, dim3(uint32_t(s.grid_count)), dim3(kBlock), 0, st, a);
+    if (s.count_planes == 4)
+        hipLaunchKernelGGL(k_bulk_count<4>, dim3(uint32_t(s.grid_count)), dim3(kBlock), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_bulk_count<8>, dim3(uint32_t(s.grid_count)), dim3(kBlock), 0, st, a);
     hipLaunchKernelGGL(k_bulk_scan, dim3(uint32_t(s.grid_scan)), dim3(kBlock), 0, st, a);
     if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());

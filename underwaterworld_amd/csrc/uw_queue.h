@@ -59,7 +59,8 @@ struct uw_bulk_shape {
     int32_t nx_tab;  // x entries per octave in the density kernel's LDS
     size_t lds_density;
     int grid_density, grid_count, grid_scan, grid_emit;
-    int emit_planes;  // cell planes per emit workgroup: 1, 2 or 4
+    int count_planes;  // cell planes per count workgroup: 4 or 8
+    int emit_planes;   // cell planes per emit workgroup: 1, 2 or 4
     size_t dens_off, rows_off, cases_off, below_off, act_off, rec_off, bytes;
 };
 
