@@ -1018,6 +1018,7 @@ int shape_batched(uw_ctx* ctx, int32_t nchunks, BatchLaunch* bl, bool allow_bulk
     if (per_cu < 1) return uw_fail(ctx, UW_ERR_INVALID_ARG, "batched kernel does not fit on a CU with this tuning");
     bl->grid = tune.grid;
     if (bl->grid == 0) bl->grid = per_cu * ctx->num_cus;
+    if (bl->grid > nchunks) bl->grid = max(1, nchunks);  // workgroups past the last task would only exit
     // x-slab chunks keep their counting-pass densities in a per-workgroup scratch for the emission
     // pass (4 (n+1)^3 bytes of HBM traffic each way instead of a second density evaluation)
     bool slabs = false;
