@@ -174,28 +174,32 @@ def test_two_outstanding_forwards_on_one_mesher():
     assert not np.allclose(g1.grad_amp.cpu().numpy(), g2.grad_amp.cpu().numpy())
 
 
-@pytest.mark.parametrize("copies", [87, 65])  # 6 chunks x 87 > 2 x 256 CUs: one workgroup per chunk;
-def test_backward_launch_forms(copies):     # 6 x 65 in (256, 512]: two workgroups per chunk
-    # the backward splits a chunk's vertices over 4, 2 or 1 workgroups by the launch's chunk count;
-    # every form gives each chunk the gradients of the 4-way split run on the 6 chunks alone
+@pytest.mark.parametrize("copies", [1, 87, 400])
+def test_backward_launch_forms(copies):
+    # The backward cuts each chunk's vertices into pieces of 128 and gives every workgroup of its
+    # persistent grid ceil(pieces / grid) consecutive pieces. 6 chunks: one piece per workgroup;
+    # 6 x 87 chunks of 8^3: still about one; 6 x 400: several per workgroup, so workgroups walk
+    # across chunk boundaries and leave one partial sum per chunk they touch. Every form must give
+    # each chunk the gradients of the 6-chunk launch (f64 sums in another order: 1e-12).
     import torch
     from underwaterworld_amd import Perlin
     from underwaterworld_amd.adjoint import ChunkMesher
     p = PR.preset("adjoint24")
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    nsub = lambda n: 4 if n <= cus else (2 if n <= 2 * cus else 1)  # noqa: E731  (uw_adjoint.hip)
-    big = COORDS * copies
-    assert nsub(len(COORDS)) == 4 and nsub(len(big)) == (1 if copies == 87 else 2), (cus, len(big))
     runs = []
-    for cs in (COORDS, big):
+    for cs in (COORDS, COORDS * copies):
         m = ChunkMesher(Perlin(p), cs, 8, want_dens_grad=True)
         rec = m.forward()
         g = m.backward(_grad_z(m, rec), record=rec)
         torch.cuda.synchronize()
-        runs.append((g.grad_amp.cpu().numpy(), g.grad_off.cpu().numpy(), g.grad_dens.cpu().numpy()))
-    (a0, o0, d0), (a1, o1, d1) = runs
+        nv = rec.batch.info()["n_verts"].astype(np.int64)
+        runs.append((g.grad_amp.cpu().numpy(), g.grad_off.cpu().numpy(), g.grad_dens.cpu().numpy(),
+                     int(((nv + 127) // 128).sum())))
+    (a0, o0, d0, _), (a1, o1, d1, pieces) = runs
+    if copies == 400:  # more pieces than the grid has workgroups (at most 8 per CU)
+        assert pieces > 8 * cus, (pieces, cus)
     assert np.any(a0 != 0)
-    for k in range(len(big)):
+    for k in range(len(COORDS) * copies):
         j = k % len(COORDS)
         assert np.allclose(a1[k], a0[j], rtol=1e-12, atol=1e-300), k
         assert np.allclose(o1[k], o0[j], rtol=1e-12, atol=1e-300), k
