@@ -162,21 +162,6 @@ __global__ void __launch_bounds__(256) k_backward_tables(BackArgs a, BackTables<
     }
 }
 
-__device__ __forceinline__ double wave_sum_f64(double v) {
-    unsigned long long u = __double_as_longlong(v);
-#define UW_F64_STEP(CTRL, MASK)                                                                   \
-    v += __longlong_as_double(static_cast<long long>(uw_dpp_u64<CTRL, MASK>(u)));                \
-    u = static_cast<unsigned long long>(__double_as_longlong(v));
-    UW_F64_STEP(UW_DPP_ROW_SHR(1), 0xF)
-    UW_F64_STEP(UW_DPP_ROW_SHR(2), 0xF)
-    UW_F64_STEP(UW_DPP_ROW_SHR(4), 0xF)
-    UW_F64_STEP(UW_DPP_ROW_SHR(8), 0xF)
-    UW_F64_STEP(UW_DPP_ROW_BCAST15, 0xA)
-    UW_F64_STEP(UW_DPP_ROW_BCAST31, 0xC)
-#undef UW_F64_STEP
-    return __longlong_as_double(static_cast<long long>(uw_wave_last(u)));
-}
-
 template <int OMAX, int OEXACT>
 __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     __shared__ float G3[3 * 256];
@@ -215,7 +200,7 @@ __global__ void __launch_bounds__(kBlock) k_backward(BackArgs a) {
     auto flush = [&](int32_t c) {
 #pragma unroll
         for (int k = 0; k < OMAX + 3; ++k) {
-            const double t = wave_sum_f64(acc[k]);
+            const double t = uw_wave_sum_f64(acc[k]);
             if ((tid & 63) == 0) red[wave][k] = t;
             acc[k] = 0.0;
         }

@@ -639,17 +639,27 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_backward(VolArgs a, const uw_
     }
 }
 
+// One workgroup per parameter: each thread adds a stride of the workgroup sums, then the waves'
+// totals are added in wave order (a fixed order, so runs agree). A single thread walking all
+// workgroups' sums in turn waited on one load per sum: 0.64 ms for 2048 workgroups.
 template <int OMAX>
-__global__ void __launch_bounds__(64) k_vol_backward_sum(const double* __restrict__ part, int32_t nblocks, int32_t O,
-                                                         double* grad_amp, double* grad_off) {
-    const int k = threadIdx.x;
-    if (k >= OMAX + 3) return;
+__global__ void __launch_bounds__(kVolBlock) k_vol_backward_sum(const double* __restrict__ part, int32_t nblocks,
+                                                                int32_t O, double* grad_amp, double* grad_off) {
+    __shared__ double ws[kVolBlock / 64];
+    const int k = int(blockIdx.x), tid = threadIdx.x;
     double t = 0.0;
-    for (int32_t b = 0; b < nblocks; ++b) t += part[size_t(b) * kVolBwdParts + k];
-    if (k < OMAX) {
-        if (k < O && grad_amp) grad_amp[k] = t;
-    } else if (grad_off) {
-        grad_off[k - OMAX] = t;
+    for (int32_t b = tid; b < nblocks; b += kVolBlock) t += part[size_t(b) * kVolBwdParts + k];
+    t = uw_wave_sum_f64(t);
+    if ((tid & 63) == 0) ws[tid >> 6] = t;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int w = 0; w < kVolBlock / 64; ++w) s += ws[w];
+        if (k < OMAX) {
+            if (k < O && grad_amp) grad_amp[k] = s;
+        } else if (grad_off) {
+            grad_off[k - OMAX] = s;
+        }
     }
 }
 
@@ -825,15 +835,18 @@ int uw_volume_backward(uw_ctx* ctx, int32_t cx, int32_t cy, int32_t cz, int32_t 
     if (O <= 4) {
         hipLaunchKernelGGL(k_vol_backward<4>, dim3(blocks), dim3(kVolBlock), 0, st, a, info, tape, grad_verts, grad_dens,
                            want, part);
-        if (want) hipLaunchKernelGGL(k_vol_backward_sum<4>, dim3(1), dim3(64), 0, st, part, blocks, O, grad_amp, grad_off);
+        if (want) hipLaunchKernelGGL(k_vol_backward_sum<4>, dim3(4 + 3), dim3(kVolBlock), 0, st, part, blocks, O, grad_amp,
+                                   grad_off);
     } else if (O <= 8) {
         hipLaunchKernelGGL(k_vol_backward<8>, dim3(blocks), dim3(kVolBlock), 0, st, a, info, tape, grad_verts, grad_dens,
                            want, part);
-        if (want) hipLaunchKernelGGL(k_vol_backward_sum<8>, dim3(1), dim3(64), 0, st, part, blocks, O, grad_amp, grad_off);
+        if (want) hipLaunchKernelGGL(k_vol_backward_sum<8>, dim3(8 + 3), dim3(kVolBlock), 0, st, part, blocks, O, grad_amp,
+                                   grad_off);
     } else {
         hipLaunchKernelGGL(k_vol_backward<16>, dim3(blocks), dim3(kVolBlock), 0, st, a, info, tape, grad_verts,
                            grad_dens, want, part);
-        if (want) hipLaunchKernelGGL(k_vol_backward_sum<16>, dim3(1), dim3(64), 0, st, part, blocks, O, grad_amp, grad_off);
+        if (want) hipLaunchKernelGGL(k_vol_backward_sum<16>, dim3(16 + 3), dim3(kVolBlock), 0, st, part, blocks, O, grad_amp,
+                                   grad_off);
     }
     if (int rc = uw_prof_end(ctx, st)) return rc;
     UW_HIP(ctx, hipGetLastError());

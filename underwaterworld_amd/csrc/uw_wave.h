@@ -52,3 +52,20 @@ __device__ __forceinline__ unsigned long long uw_wave_last(unsigned long long v)
     const uint32_t lo = uw_wave_last(uint32_t(v)), hi = uw_wave_last(uint32_t(v >> 32));
     return (static_cast<unsigned long long>(hi) << 32) | lo;
 }
+
+// Sum over the wave's 64 lanes of an f64 value (the inclusive scan's last lane), wave-uniform; the
+// addition order is fixed, so the result does not vary between runs.
+__device__ __forceinline__ double uw_wave_sum_f64(double v) {
+    unsigned long long u = static_cast<unsigned long long>(__double_as_longlong(v));
+#define UW_F64_STEP(CTRL, MASK)                                                                   \
+    v += __longlong_as_double(static_cast<long long>(uw_dpp_u64<CTRL, MASK>(u)));                \
+    u = static_cast<unsigned long long>(__double_as_longlong(v));
+    UW_F64_STEP(UW_DPP_ROW_SHR(1), 0xF)
+    UW_F64_STEP(UW_DPP_ROW_SHR(2), 0xF)
+    UW_F64_STEP(UW_DPP_ROW_SHR(4), 0xF)
+    UW_F64_STEP(UW_DPP_ROW_SHR(8), 0xF)
+    UW_F64_STEP(UW_DPP_ROW_BCAST15, 0xA)
+    UW_F64_STEP(UW_DPP_ROW_BCAST31, 0xC)
+#undef UW_F64_STEP
+    return __longlong_as_double(static_cast<long long>(uw_wave_last(u)));
+}
