@@ -177,6 +177,27 @@ __device__ __forceinline__ void load_plane(const VolArgs& a, int32_t x, int32_t 
     }
 }
 
+// The same plane through registers (kPlaneRegs floats per thread): the loads are issued one plane
+// ahead and their registers stored into LDS when the plane's slot is free.
+constexpr int kPlaneRegs = (kTileS * kTileS + kVolBlock - 1) / kVolBlock;
+__device__ __forceinline__ void fetch_plane(const VolArgs& a, int32_t x, int32_t ys, int32_t sy, int32_t zs, int32_t sz,
+                                            float (&r)[kPlaneRegs]) {
+    const float* src = a.dens + size_t(x) * a.L * a.L;
+#pragma unroll
+    for (int i = 0; i < kPlaneRegs; ++i) {
+        const int e = int(threadIdx.x) + i * kVolBlock;
+        const int zz = e / sy, yy = e - zz * sy;
+        r[i] = e < sy * sz ? src[size_t(zs + zz) * a.L + ys + yy] : 0.0f;
+    }
+}
+__device__ __forceinline__ void put_plane(int32_t sy, int32_t sz, const float (&r)[kPlaneRegs], float* dst) {
+#pragma unroll
+    for (int i = 0; i < kPlaneRegs; ++i) {
+        const int e = int(threadIdx.x) + i * kVolBlock;
+        const int zz = e / sy, yy = e - zz * sy;
+        if (e < sy * sz) dst[zz * kPitch + yy] = r[i];
+    }
+}
 // Corner densities of cell (x, y, z) from the LDS sample planes p0 (plane x) and p1 (plane x + 1)
 // whose origin is (ys, zs).
 __device__ __forceinline__ void tile_corners(const float* p0, const float* p1, int32_t y, int32_t z, int32_t ys,
@@ -346,12 +367,17 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_scan_apply(unsigned long long
 // Per emitted plane the active cells are compacted into a list (LDS atomics per row, any order:
 // every id and index position comes from the row bases), then one lane per listed cell writes its
 // vertices and one lane per triangle its three index entries, so no lane idles on an empty cell or
-// waits for the cell with the most triangles.
+// waits for the cell with the most triangles. The slab's row bases are read into LDS once, and each
+// sample plane is fetched into registers while the plane before it is processed, so no memory
+// round trip sits between two planes.
 constexpr int kT2C = 2048;  // triangles of a plane taken one per lane (more: one lane per cell)
 
 __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
     __shared__ float SP[2][kTileS * kPitch];
-    __shared__ uint32_t CB[2][(kTile + 1) * (kTile + 1)];  // vertex id of the cell's first owned record
+    // first owned record of the cell, relative to its row segment's base (the halo column's cells,
+    // which end the segment below, hold minus their own records)
+    __shared__ int16_t CB[2][(kTile + 1) * (kTile + 1)];
+    __shared__ unsigned long long RB[(kEmitPlanes + 1) * (kTile + 1)];  // row bases of planes xs .. X1 - 1
     __shared__ uint8_t CC[2][(kTile + 1) * (kTile + 1)];   // cube case
     __shared__ uint32_t LIST[kTile * kTile];  // active cells of the plane: (y - Y0) 32 + z - Z0 | triangle prefix << 10
     __shared__ uint32_t LIB[kTile * kTile];   // index position of each listed cell's first index entry
@@ -378,14 +404,25 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
     const uw_emit_opts& eo = a.eo;
     const bool coll = eo.tri_normals || eo.tri_cells;
     const uint64_t* __restrict__ tri_rows = a.tab->tri;
-    load_plane(a, xs, ys, sy, zs, sz, SP[xs & 1]);
+    static_assert((kEmitPlanes + 1) * (kTile + 1) <= kVolBlock, "one row base per thread");
+    if (tid < (X1 - xs) * ny) {
+        const int32_t pl = tid / ny, r = tid - pl * ny;
+        RB[pl * (kTile + 1) + r] = a.rows[(size_t(xs + pl) * n + ys + r) * a.T + tz];
+    }
+    // row base of row r of cell plane x
+    auto rbase = [&](int32_t x, int32_t r) -> unsigned long long { return RB[(x - xs) * (kTile + 1) + r]; };
+    float pre[kPlaneRegs];
+    fetch_plane(a, xs, ys, sy, zs, sz, pre);
+    put_plane(sy, sz, pre, SP[xs & 1]);
+    fetch_plane(a, xs + 1, ys, sy, zs, sz, pre);
     for (int32_t x = xs; x < X1; ++x) {
-        load_plane(a, x + 1, ys, sy, zs, sz, SP[(x + 1) & 1]);
+        put_plane(sy, sz, pre, SP[(x + 1) & 1]);
+        if (x + 2 <= X1) fetch_plane(a, x + 2, ys, sy, zs, sz, pre);  // block-uniform
         if (tid == 0) s_cnt = 0u;
         __syncthreads();
         const float* p0 = SP[x & 1];
         const float* p1 = SP[(x + 1) & 1];
-        uint32_t* cb = CB[x & 1];
+        int16_t* cb = CB[x & 1];
         uint8_t* cc = CC[x & 1];
         // classification, vertex and index bases of the plane's rows (cells z >= Z0 of a row lie
         // in the row segment (x, y, tz); the halo cell z = Z0 - 1 is the last of segment tz - 1),
@@ -412,10 +449,10 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
             const uint32_t lbase = (lane & 32) ? uint32_t(__builtin_amdgcn_readlane(int(old), 63))
                                                : uint32_t(__builtin_amdgcn_readlane(int(old), 31));
             if (in) {
-                const unsigned long long rb = a.rows[(size_t(x) * n + y) * a.T + tz];
+                const unsigned long long rb = rbase(x, r);
                 const uint32_t ex = inc - v;
                 const int o = r * nz + (z - zs);
-                cb[o] = uint32_t(rb) + (ex & 0x3FFu);
+                cb[o] = int16_t(ex & 0x3FFu);
                 cc[o] = uint8_t(cs);
                 if (emitted && (v >> 20)) {
                     const uint32_t slot = (lbase & 0xFFFFu) + (ex >> 20);
@@ -431,8 +468,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
                 tile_corners(p0, p1, y, z, ys, zs, rr);
                 const uint32_t cs = uw_case_of(rr, iso);
                 const uint32_t nv = __popc(LUT[cs] & 0xFFFu & uw_owned_mask(x, y, z));
-                const unsigned long long rb = a.rows[(size_t(x) * n + y) * a.T + tz];
-                cb[r * nz] = uint32_t(rb) - nv;
+                cb[r * nz] = int16_t(-int32_t(nv));
                 cc[r * nz] = uint8_t(cs);
             }
         }
@@ -449,7 +485,8 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
                 uw_edge_owner(x, y, z, ed, &qx, &qy, &qz, &qe);
                 const int qo = (qy - ys) * nz + (qz - zs);
                 const uint32_t qc = CC[qx & 1][qo];
-                return CB[qx & 1][qo] + __popc(uint32_t(BEF[qc * 12 + qe]) & uw_owned_mask(qx, qy, qz));
+                return uint32_t(rbase(qx, qy - ys)) + uint32_t(int32_t(CB[qx & 1][qo])) +
+                       __popc(uint32_t(BEF[qc * 12 + qe]) & uw_owned_mask(qx, qy, qz));
             };
             // index entries (and collision records) of triangle j of a listed cell
             auto emit_tri = [&](int32_t y, int32_t z, uint32_t cs, uint32_t owned, uint32_t myv, uint64_t row, uint32_t j,
@@ -482,7 +519,7 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
                 float r[8];
                 tile_corners(p0, p1, y, z, ys, zs, r);
                 const uint32_t owned = uw_owned_mask(x, y, z);
-                const uint32_t myv = cb[o];
+                const uint32_t myv = uint32_t(rbase(x, y - ys)) + uint32_t(int32_t(cb[o]));
                 for (uint32_t vm = LUT[cs] & 0xFFFu & owned; vm; vm &= vm - 1u) {
                     const uint32_t ed = uint32_t(__builtin_ctz(vm));
                     const uint64_t id = uint64_t(myv) + __popc(uint32_t(BEF[cs * 12 + ed]) & owned);
@@ -527,7 +564,8 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_emit(VolArgs a) {
                     const uint32_t j = t - (ent >> 10);
                     float r[8];
                     if (coll) tile_corners(p0, p1, y, z, ys, zs, r);
-                    emit_tri(y, z, cs, uw_owned_mask(x, y, z), cb[o], tri_rows[cs], j, uint64_t(LIB[k]) + 3 * j, r);
+                    emit_tri(y, z, cs, uw_owned_mask(x, y, z), uint32_t(rbase(x, y - ys)) + uint32_t(int32_t(cb[o])),
+                             tri_rows[cs], j, uint64_t(LIB[k]) + 3 * j, r);
                 }
             }
         }
