@@ -156,11 +156,14 @@ __global__ void __launch_bounds__(kBlock) k_bulk_density(BulkArgs a) {
         float* out = a.dens + uint64_t(j) * slot + (size_t(x) * L) * L + y;
         float vmin = INFINITY, vmax = -INFINITY;
         const int32_t nxt = a.nx_tab;
-        uw_noise_zline<OCT, false>(
-            a.f, [=](int k) { return ax[k * nxt]; }, [=](int k) { return ay[k * kBlock]; }, P, G3, Zt.rec, rw, L,
-            vmin, vmax, [=](int32_t iz, float rho) {
-                if (valid) out[iz * L] = rho;
-            });
+        // a wave with no line of the chunk (the last item of a chunk: 65 of 256 lines at 32^3, 113
+        // at 24^3) skips the walk, leaving its SIMD to the other waves
+        if (__any(valid))
+            uw_noise_zline<OCT, false>(
+                a.f, [=](int k) { return ax[k * nxt]; }, [=](int k) { return ay[k * kBlock]; }, P, G3, Zt.rec, rw, L,
+                vmin, vmax, [=](int32_t iz, float rho) {
+                    if (valid) out[iz * L] = rho;
+                });
         const int bits = valid ? uw_below_bits(vmin, vmax, a.f.iso) : 0;
         if (bits) atomicOr(&s_bits, uint32_t(bits));
         __syncthreads();
