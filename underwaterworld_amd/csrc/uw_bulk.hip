@@ -8,10 +8,11 @@
 //   k_bulk_count    one workgroup per (chunk, x-slab of 8 or 4 cell planes): cube cases and, per row
 //                   (x, y), the owned vertex records and triangles (chunk.rs:141-164 with the
 //                   ownership rule)
-//   k_bulk_scan     one workgroup per chunk: the chunk's row bases (first-seen order within the
-//                   chunk), its totals and record, and one reservation of its vertex and index
-//                   ranges on the launch's range counter (the persistent kernel's reservations use
-//                   the same counter, so ranges never overlap)
+//   k_bulk_scan     one wave per chunk, four per workgroup: the chunk's row bases (first-seen
+//                   order within the chunk), its totals and record; the workgroup's chunks reserve
+//                   their vertex and index ranges with one atomic on the launch's range counter
+//                   (the persistent kernel's reservations use the same counter, so ranges never
+//                   overlap)
 //   k_bulk_emit     one workgroup per (chunk, x-slab of 1, 2 or 4 cell planes): classifies the slab's planes
 //                   and the one below it (owners of shared records lie at most one plane back),
 //                   then writes the planes' vertices, indices and, when requested, tape, colours
@@ -303,75 +304,109 @@ __global__ void __launch_bounds__(kBlock) k_bulk_count(BulkArgs a) {
 }
 
 // ---------------------------------------------------------------------------------------------
-__device__ __forceinline__ unsigned long long bulk_block_scan(unsigned long long v, unsigned long long* total) {
-    __shared__ unsigned long long ws[kBlock / 64];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const unsigned long long inc = uw_wave_incl_scan(v);
-    if (lane == 63) ws[wave] = inc;
-    __syncthreads();
-    unsigned long long base = 0, sum = 0;
-#pragma unroll
-    for (int w = 0; w < kBlock / 64; ++w) {
-        base += w < wave ? ws[w] : 0ull;
-        sum += ws[w];
-    }
-    __syncthreads();
-    *total = sum;
-    return base + inc - v;
-}
-
+// One wave per bulk entry, four entries per workgroup pass: the wave scans its chunk's rows (16
+// per lane, then a DPP wave scan), and the workgroup's chunks reserve their vertex and index ranges
+// with one atomic for their sum, each chunk's range following from its prefix among them. The
+// range counter is the launch's one contended address (the memory-side atomic unit serialises it):
+// one reservation per chunk had made this kernel 41 us for the mixed set's 4,096 bulk chunks.
 __global__ void __launch_bounds__(kBlock) k_bulk_scan(BulkArgs a) {
-    const int tid = threadIdx.x;
+    constexpr int W = kBlock / 64;
+    constexpr int kPerLane = (kMaxN * kMaxN + 63) / 64;  // rows per lane
+    __shared__ unsigned long long s_tot[W];  // per wave: its chunk's vertices | indices << 32 (0: none)
+    __shared__ unsigned long long s_vo, s_io;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const uint32_t nbig = bulk_count(a);
     unsigned long long* resv = uw_queue_resv(a.qhead);
-    constexpr int kPer = (kMaxN * kMaxN + kBlock - 1) / kBlock;  // rows per thread
-    for (uint32_t j = blockIdx.x; j < nbig; j += gridDim.x) {
-        int32_t c, n, cx, cy, cz;
-        load_task(a, j, &c, &n, &cx, &cy, &cz);
-        unsigned long long* rec = a.rec + 4 * uint64_t(j);
-        if (n < 1 || n > a.max_res) {
-            if (tid == 0) {
+    for (uint32_t j0 = blockIdx.x * uint32_t(W); j0 < nbig; j0 += gridDim.x * uint32_t(W)) {  // block-uniform
+        const uint32_t j = j0 + uint32_t(wave);
+        int32_t c = 0, n = 0, cx, cy, cz;
+        bool valid = false;
+        unsigned long long tot = 0;
+        if (j < nbig) {  // wave-uniform
+            load_task(a, j, &c, &n, &cx, &cy, &cz);
+            valid = n >= 1 && n <= a.max_res;
+            if (valid) {
+                const int32_t R = n * n;
+                unsigned long long* rows = a.rows + uint64_t(j) * a.max_res * a.max_res;
+                unsigned long long v[kPerLane];
+                unsigned long long sum = 0;
+#pragma unroll
+                for (int k = 0; k < kPerLane; ++k) {
+                    const int r = lane * kPerLane + k;
+                    v[k] = r < R ? rows[r] : 0ull;
+                    sum += v[k];
+                }
+                const unsigned long long inc = uw_wave_incl_scan(sum);
+                unsigned long long ex = inc - sum;
+#pragma unroll
+                for (int k = 0; k < kPerLane; ++k) {
+                    const int r = lane * kPerLane + k;
+                    if (r < R) rows[r] = ex;
+                    ex += v[k];
+                }
+                tot = uw_wave_last(inc);  // vertex records | triangles << 32
+            }
+        }
+        const uint32_t nv = uint32_t(tot & 0xFFFFFFFFull), ni = 3u * uint32_t(tot >> 32);
+        if (lane == 0) s_tot[wave] = ni > 0 ? ((unsigned long long)nv | ((unsigned long long)ni << 32)) : 0ull;
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long all = 0;
+            uint32_t nonblank = 0;
+#pragma unroll
+            for (int w = 0; w < W; ++w) {
+                all += s_tot[w];
+                nonblank += s_tot[w] ? 1u : 0u;
+            }
+            unsigned long long vo = 0, io = 0;
+            if (all) {
+                const uint32_t av = uint32_t(all & 0xFFFFFFFFull), ai = uint32_t(all >> 32);
+                if (a.packed) {
+                    const unsigned long long old = atomicAdd(resv, all);
+                    vo = old & 0xFFFFFFFFull;
+                    io = old >> 32;
+                } else {
+                    vo = atomicAdd(resv, (unsigned long long)av);
+                    io = atomicAdd(resv + kResvIndsOffset, (unsigned long long)ai);
+                }
+                atomicAdd(resv + kExitOffset, (unsigned long long)nonblank << 32);  // non-blank chunks of the launch
+            }
+            s_vo = vo;
+            s_io = io;
+        }
+        __syncthreads();
+        if (j < nbig && lane == 0) {
+            unsigned long long* rec = a.rec + 4 * uint64_t(j);
+            if (!valid) {
                 uw_chunk_info r = {UW_CHUNK_INVALID | UW_CHUNK_BLANK, 0, 0, 0, 0, 0};
                 a.info[c] = r;
                 rec[0] = ~0ull;
+            } else {
+                unsigned long long pre = 0;
+                for (int w = 0; w < wave; ++w) pre += s_tot[w];
+                uint32_t flags = 0;
+                unsigned long long vo = 0, io = 0;
+                if (!(a.below[j] & UW_BELOW_NOT_ABOVE)) flags = UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP;
+                else if (ni == 0) flags = UW_CHUNK_BLANK;
+                if (ni > 0) {
+                    vo = s_vo + (pre & 0xFFFFFFFFull);
+                    io = s_io + (pre >> 32);
+                    if (vo + nv > a.vcap || io + ni > a.icap) {
+                        flags |= UW_CHUNK_OVERFLOW;
+                        atomicOr(&a.totals->overflow, 1u);
+                    }
+                }
+                if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
+                uw_chunk_info r = {flags, nv, ni, a.act[j], vo, io};
+                a.info[c] = r;
+                const bool emit = ni > 0 && !(flags & UW_CHUNK_OVERFLOW);
+                rec[0] = emit ? vo : ~0ull;
+                rec[1] = io;
+                rec[2] = nv;
+                rec[3] = ni;
             }
-            continue;
         }
-        const int32_t R = n * n;
-        unsigned long long* rows = a.rows + uint64_t(j) * a.max_res * a.max_res;
-        unsigned long long v[kPer];
-        unsigned long long s = 0;
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int r = tid * kPer + k;
-            v[k] = r < R ? rows[r] : 0ull;
-            s += v[k];
-        }
-        unsigned long long tot;
-        unsigned long long ex = bulk_block_scan(s, &tot);
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int r = tid * kPer + k;
-            if (r < R) rows[r] = ex;
-            ex += v[k];
-        }
-        if (tid == 0) {
-            const uint32_t nv = uint32_t(tot & 0xFFFFFFFFull), ni = 3u * uint32_t(tot >> 32);
-            uint32_t flags = 0;
-            unsigned long long vo = 0, io = 0;
-            if (!(a.below[j] & UW_BELOW_NOT_ABOVE)) flags = UW_CHUNK_BLANK | UW_CHUNK_EARLY_SKIP;
-            else if (ni == 0) flags = UW_CHUNK_BLANK;
-            if (ni > 0) uw_reserve_ranges(resv, a.packed, a.vcap, a.icap, a.totals, nv, ni, &vo, &io, &flags);
-            if (a.eo.ind16 && nv > 65535u) flags |= UW_CHUNK_INDEX_WRAP;
-            uw_chunk_info r = {flags, nv, ni, a.act[j], vo, io};
-            a.info[c] = r;
-            const bool emit = ni > 0 && !(flags & UW_CHUNK_OVERFLOW);
-            rec[0] = emit ? vo : ~0ull;
-            rec[1] = io;
-            rec[2] = nv;
-            rec[3] = ni;
-            if (ni > 0) atomicAdd(resv + kExitOffset, 1ull << 32);  // non-blank chunks of the launch
-        }
+        __syncthreads();  // s_tot, s_vo and s_io are written again in the next pass
     }
 }
 
