@@ -126,6 +126,9 @@ __device__ __forceinline__ int order_bucket(const uw_field& f, const uint8_t* P,
 //           slot c of the bulk path's per-entry counters (early-blank bits, active cells; indexed
 //           by queue entry, so clearing every slot below nchunks covers the bulk entries; null
 //           without the bulk path).
+// A launch of at most kPrepThreads chunks has one estimate workgroup, whose histogram is the
+// launch's: it places its chunks and does workgroup 0's resets itself, and k_place is not launched
+// (one short launch, ~4 us, less for such launches).
 // The histograms are written whole (zeros included) by every launch, so no state carries over.
 // Per-chunk key: bucket | rank << 16 (rank < kPrepThreads).
 constexpr int kPrepThreads = 256;
@@ -145,10 +148,56 @@ inline OrderBufs order_bufs(int32_t nchunks) {
     return o;
 }
 
+// Queue placement and resets (k_place's work), passed to k_prep so that a launch of one estimate
+// workgroup (at most kPrepThreads chunks) places its chunks in the same launch.
+struct PlaceArgs {
+    uw_task* tasks;
+    uint32_t* head;
+    uw_totals* totals;
+    uint32_t first_task, bulk_bucket;
+    uint32_t* bulk_below;
+    uint32_t* bulk_act;
+};
+
+// Workgroup 0's resets of k_place: range and exit counters, launch totals.
+__device__ __forceinline__ void reset_queue(const PlaceArgs& pa) {
+    unsigned long long* resv = uw_queue_resv(pa.head);
+    resv[0] = 0ull;
+    resv[kResvIndsOffset] = 0ull;
+    resv[kExitOffset] = 0ull;
+    uw_totals z = {0, 0, 0, 0};
+    *pa.totals = z;
+}
+
+// Task record of chunk c at queue slot `slot`, and the clearing of its bulk counters.
+__device__ __forceinline__ void place_chunk(const PlaceArgs& pa, int32_t c, int32_t n, int32_t cx, int32_t cy,
+                                            int32_t cz, uint32_t slot) {
+    if (pa.bulk_below) {
+        pa.bulk_below[c] = 0u;
+        pa.bulk_act[c] = 0u;
+    }
+    uw_task t;
+    t.c = c;
+    t.n = n;
+    t.cx = cx;
+    t.cy = cy;
+    t.cz = cz;
+    t.pad[0] = t.pad[1] = t.pad[2] = 0;
+    pa.tasks[slot] = t;
+}
+
+// Bulk entry count and queue head from the global bucket offsets (one thread).
+__device__ __forceinline__ void publish_head(const PlaceArgs& pa, const uint32_t* base) {
+    const uint32_t nbig =
+        (pa.bulk_bucket > 0u && pa.bulk_bucket < uint32_t(kOrderBuckets)) ? base[pa.bulk_bucket] : 0u;
+    *uw_queue_bulk_count(pa.head) = nbig;
+    *pa.head = nbig + pa.first_task;
+}
+
 __device__ __forceinline__ void estimate_block(const uw_field& f, const uw_tables* __restrict__ tab,
                                                const int32_t* __restrict__ coords, const int32_t* __restrict__ res,
                                                int32_t nchunks, uint32_t* __restrict__ keys,
-                                               uint32_t* __restrict__ hist) {
+                                               uint32_t* __restrict__ hist, const PlaceArgs* place) {
     __shared__ uint32_t cnt[kOrderBuckets];
     __shared__ uint8_t P[256];
     __shared__ float G3[3 * 256];
@@ -177,6 +226,31 @@ __device__ __forceinline__ void estimate_block(const uw_field& f, const uw_table
     const int bk = order_bucket(f, P, G3, n, cx, cy, cz);
     const uint32_t rank = valid ? atomicAdd(&cnt[bk], 1u) : 0u;
     __syncthreads();
+    if (place) {
+        // the launch's only estimate workgroup: its histogram is the launch's; place the chunks
+        // here (exclusive prefix of the bucket counts, three buckets per thread, wave scans)
+        __shared__ uint32_t wsum[kPrepThreads / 64];
+        uint32_t t3[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) t3[k] = 3 * tid + k < kOrderBuckets ? cnt[3 * tid + k] : 0u;
+        const uint32_t mine = t3[0] + t3[1] + t3[2];
+        const uint32_t inc = uw_wave_incl_scan(mine);
+        if ((tid & 63) == 63) wsum[tid >> 6] = inc;
+        __syncthreads();
+        uint32_t off = inc - mine;
+        for (int w = 0; w < (tid >> 6); ++w) off += wsum[w];
+        __syncthreads();  // every count is read: cnt[] takes the offsets
+        if (3 * tid < kOrderBuckets) cnt[3 * tid] = off;
+        if (3 * tid + 1 < kOrderBuckets) cnt[3 * tid + 1] = off + t3[0];
+        if (3 * tid + 2 < kOrderBuckets) cnt[3 * tid + 2] = off + t3[0] + t3[1];
+        __syncthreads();
+        if (tid == 0) {
+            reset_queue(*place);
+            publish_head(*place, cnt);
+        }
+        if (valid) place_chunk(*place, c, n, cx, cy, cz, cnt[bk] + rank);
+        return;
+    }
     uint32_t* h = hist + size_t(blockIdx.x) * kOrderBuckets;
     for (int k = tid; k < kOrderBuckets; k += kPrepThreads) h[k] = cnt[k];
     if (valid) keys[c] = uint32_t(bk) | (rank << 16);
@@ -188,9 +262,10 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const uw_tabl
                                                        const int32_t* __restrict__ coords,
                                                        const int32_t* __restrict__ res, int32_t nchunks,
                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ hist,
-                                                       int32_t nest, int32_t stride, unsigned char* __restrict__ ztab) {
+                                                       int32_t nest, int32_t stride, unsigned char* __restrict__ ztab,
+                                                       PlaceArgs pa) {
     if (int32_t(blockIdx.x) < nest) {
-        estimate_block(f, tab, coords, res, nchunks, keys, hist);
+        estimate_block(f, tab, coords, res, nchunks, keys, hist, nest == 1 ? &pa : nullptr);
         return;
     }
     const uint64_t g = uint64_t(blockIdx.x - uint32_t(nest)) * kPrepThreads + threadIdx.x;
@@ -200,22 +275,12 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const uw_tabl
 __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restrict__ coords,
                                                         const int32_t* __restrict__ res, int32_t nchunks,
                                                         const uint32_t* __restrict__ keys,
-                                                        const uint32_t* __restrict__ hist, int32_t nest,
-                                                        uw_task* __restrict__ tasks, uint32_t* head, uw_totals* totals,
-                                                        uint32_t first_task, uint32_t bulk_bucket,
-                                                        uint32_t* __restrict__ bulk_below, uint32_t* __restrict__ bulk_act) {
+                                                        const uint32_t* __restrict__ hist, int32_t nest, PlaceArgs pa) {
     __shared__ uint32_t base[3 * kPrepThreads];
     __shared__ uint32_t wsum[kPrepThreads / 64];
     const int tid = threadIdx.x;
     const int32_t j = int32_t(blockIdx.x);
-    if (j == 0 && tid == 0) {
-        unsigned long long* resv = uw_queue_resv(head);
-        resv[0] = 0ull;
-        resv[kResvIndsOffset] = 0ull;
-        resv[kExitOffset] = 0ull;
-        uw_totals z = {0, 0, 0, 0};
-        *totals = z;
-    }
+    if (j == 0 && tid == 0) reset_queue(pa);
     const int32_t c = j * kPrepThreads + tid;
     const bool valid = c < nchunks;
     const uint32_t key = valid ? keys[c] : 0u;
@@ -258,25 +323,8 @@ __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restric
     base[3 * tid + 1] = off + tot[0] + before[1];
     base[3 * tid + 2] = off + tot[0] + tot[1] + before[2];
     __syncthreads();
-    if (j == 0 && tid == 0) {  // workgroup 0 has no workgroups before it: base[] are the global offsets
-        const uint32_t nbig = (bulk_bucket > 0u && bulk_bucket < uint32_t(kOrderBuckets)) ? base[bulk_bucket] : 0u;
-        *uw_queue_bulk_count(head) = nbig;
-        *head = nbig + first_task;
-    }
-    if (valid && bulk_below) {
-        bulk_below[c] = 0u;
-        bulk_act[c] = 0u;
-    }
-    if (valid) {
-        uw_task t;
-        t.c = c;
-        t.n = n;
-        t.cx = cx;
-        t.cy = cy;
-        t.cz = cz;
-        t.pad[0] = t.pad[1] = t.pad[2] = 0;
-        tasks[base[key & 0xFFFFu] + (key >> 16)] = t;
-    }
+    if (j == 0 && tid == 0) publish_head(pa, base);  // workgroup 0 has none before it: base[] are global
+    if (valid) place_chunk(pa, c, n, cx, cy, cz, base[key & 0xFFFFu] + (key >> 16));
 }
 
 // Reserves a chunk's contiguous vertex and index ranges (one thread). A packed launch (worst-case
@@ -1107,15 +1155,18 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     const OrderBufs ob = order_bufs(nchunks);
     uint32_t* keys = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(task_buf) + ob.keys_off);
     uint32_t* hist = reinterpret_cast<uint32_t*>(static_cast<unsigned char*>(task_buf) + ob.hist_off);
-    if (int rc = uw_prof_start(ctx, st, UW_STAGE_BATCH_SETUP)) return rc;
-    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks) + uint32_t(nest)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords,
-                       res, nchunks, keys, hist, nest, a.zstride, ztab);
     uw_bulk_io bio;
     memset(&bio, 0, sizeof(bio));
-    uint32_t bulk_bucket = 0;
-    uint32_t *bulk_below = nullptr, *bulk_act = nullptr;
+    PlaceArgs pa;
+    pa.tasks = tasks;
+    pa.head = head;
+    pa.totals = a.totals;
+    pa.first_task = uint32_t(bl.grid);
+    pa.bulk_bucket = 0;
+    pa.bulk_below = nullptr;
+    pa.bulk_act = nullptr;
     if (bl.bulk.on) {
-        bulk_bucket = uint32_t(kMaxN - ctx->batch_tuning.bulk_min + 1) * kSurfBins;
+        pa.bulk_bucket = uint32_t(kMaxN - ctx->batch_tuning.bulk_min + 1) * kSurfBins;
         bio.tasks = tasks;
         bio.qhead = head;
         bio.ztab = ztab;
@@ -1124,11 +1175,16 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
         bio.out = out;
         bio.tape = tape;
         bio.scratch = bulk_scratch;
-        bulk_below = reinterpret_cast<uint32_t*>(bulk_scratch + bl.bulk.below_off);
-        bulk_act = reinterpret_cast<uint32_t*>(bulk_scratch + bl.bulk.act_off);
+        pa.bulk_below = reinterpret_cast<uint32_t*>(bulk_scratch + bl.bulk.below_off);
+        pa.bulk_act = reinterpret_cast<uint32_t*>(bulk_scratch + bl.bulk.act_off);
     }
-    hipLaunchKernelGGL(k_place, dim3(uint32_t(nest)), dim3(kPrepThreads), 0, st, coords, res, nchunks, keys, hist, nest,
-                       tasks, head, a.totals, uint32_t(bl.grid), bulk_bucket, bulk_below, bulk_act);
+    if (int rc = uw_prof_start(ctx, st, UW_STAGE_BATCH_SETUP)) return rc;
+    // a launch of at most kPrepThreads chunks has one estimate workgroup, which also places them
+    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks) + uint32_t(nest)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords,
+                       res, nchunks, keys, hist, nest, a.zstride, ztab, pa);
+    if (nest > 1)
+        hipLaunchKernelGGL(k_place, dim3(uint32_t(nest)), dim3(kPrepThreads), 0, st, coords, res, nchunks, keys, hist,
+                           nest, pa);
     if (int rc = uw_prof_end(ctx, st)) return rc;
     // bulk chunks: densities, counts and range reservations before the persistent kernel (whose
     // last workgroup writes the launch totals from the shared range counter), emission after it
