@@ -423,7 +423,9 @@ __global__ void __launch_bounds__(kBlock) k_bulk_scan(BulkArgs a) {
 constexpr int kMapCap = 1280;  // entries of each per-plane map (LDS)
 
 template <int P>
-__global__ void __launch_bounds__(kEmitBlock) k_bulk_emit(BulkArgs a) {
+// At most 96 VGPRs (the 4-plane form spills two): five workgroups per CU, which the 30 KB of LDS
+// allows (mixed emission 0.383 -> 0.326 ms).
+__global__ void __launch_bounds__(kEmitBlock, 5) k_bulk_emit(BulkArgs a) {
     constexpr int kEmitPlanes = P;
     __shared__ float SP[2][kMaxL * kMaxL];     // sample planes x, x + 1 ([z][y], pitch L)
     __shared__ uint16_t CB[2][kMaxN * kMaxN];  // first vertex id within its row: cells of planes x - 1, x
@@ -692,12 +694,12 @@ int uw_bulk_shape_for(uw_ctx* ctx, const uw_batch_tuning& t, int32_t nchunks, uw
     s->count_planes = nc64 * ((t.max_res + 7) / 8) < 2 * int64_t(resident_count) ? 4 : 8;
     const int64_t gc = nc64 * ((t.max_res + s->count_planes - 1) / s->count_planes);
     // emit: the deepest slab (fewest halo planes) whose workgroups still cover the resident slots
-    // about four times over; a launch of few chunks gets shallow slabs, so more workgroups share
-    // its chunks
+    // about three times over; a launch of few chunks gets shallow slabs, so more workgroups share
+    // its chunks (the mixed set: 4 planes; the adjoint's 256 chunks: 2)
     int resident = 0;
     if (int rc = grid_for(ctx, reinterpret_cast<const void*>(k_bulk_emit<4>), 0, &resident, kEmitBlock)) return rc;
     s->emit_planes = 4;
-    while (s->emit_planes > 1 && nc64 * ((t.max_res + s->emit_planes - 1) / s->emit_planes) < 4 * int64_t(resident))
+    while (s->emit_planes > 1 && nc64 * ((t.max_res + s->emit_planes - 1) / s->emit_planes) < 3 * int64_t(resident))
         s->emit_planes /= 2;
     const int64_t ge = nc64 * ((t.max_res + s->emit_planes - 1) / s->emit_planes);
     if (ge > 0x7FFFFFFF) return uw_fail(ctx, UW_ERR_INVALID_ARG, "too many chunks for the bulk path");
