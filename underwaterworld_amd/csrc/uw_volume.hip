@@ -166,19 +166,9 @@ __device__ __forceinline__ uint32_t uw_half_incl_scan(uint32_t v) {
     return v;
 }
 
-// Copies sample plane x, y in [ys, ys + sy), z in [zs, zs + sz), into an LDS plane [z - zs][y - ys]
-// of pitch kPitch.
-__device__ __forceinline__ void load_plane(const VolArgs& a, int32_t x, int32_t ys, int32_t sy, int32_t zs, int32_t sz,
-                                           float* dst) {
-    const float* src = a.dens + size_t(x) * a.L * a.L;
-    for (int e = threadIdx.x; e < sy * sz; e += kVolBlock) {
-        const int zz = e / sy, yy = e - zz * sy;
-        dst[zz * kPitch + yy] = src[size_t(zs + zz) * a.L + ys + yy];
-    }
-}
-
-// The same plane through registers (kPlaneRegs floats per thread): the loads are issued one plane
-// ahead and their registers stored into LDS when the plane's slot is free.
+// Sample plane x, y in [ys, ys + sy), z in [zs, zs + sz), through registers (kPlaneRegs floats per
+// thread) into an LDS plane [z - zs][y - ys] of pitch kPitch: the loads are issued one plane ahead
+// and their registers stored into LDS when the plane's slot is free.
 constexpr int kPlaneRegs = (kTileS * kTileS + kVolBlock - 1) / kVolBlock;
 __device__ __forceinline__ void fetch_plane(const VolArgs& a, int32_t x, int32_t ys, int32_t sy, int32_t zs, int32_t sz,
                                             float (&r)[kPlaneRegs]) {
@@ -247,11 +237,16 @@ __global__ void __launch_bounds__(kVolBlock) k_vol_count(VolArgs a) {
     for (int i = tid; i < 256; i += kVolBlock) LUT[i] = uint32_t(a.tab->used[i]) | (uint32_t(a.tab->tri[i] & 15u) << 16);
     if (tid == 0) s_act = 0u;
     const int32_t sy = Y1 - Y0 + 1, sz = Z1 - Z0 + 1;
-    load_plane(a, X0, Y0, sy, Z0, sz, SP[X0 & 1]);
+    // sample plane x + 2 is fetched into registers while plane x is classified
+    float pre[kPlaneRegs];
+    fetch_plane(a, X0, Y0, sy, Z0, sz, pre);
+    put_plane(sy, sz, pre, SP[X0 & 1]);
+    fetch_plane(a, X0 + 1, Y0, sy, Z0, sz, pre);
     const float iso = a.f.iso;
     uint32_t act = 0;
     for (int32_t x = X0; x < X1; ++x) {
-        load_plane(a, x + 1, Y0, sy, Z0, sz, SP[(x + 1) & 1]);
+        put_plane(sy, sz, pre, SP[(x + 1) & 1]);
+        if (x + 2 <= X1) fetch_plane(a, x + 2, Y0, sy, Z0, sz, pre);  // block-uniform
         __syncthreads();
         const float* p0 = SP[x & 1];
         const float* p1 = SP[(x + 1) & 1];
