@@ -154,7 +154,7 @@ enum uw_profile_stage {
     UW_STAGE_PLAIN_CLASSIFY = 1, /* k_classify: cases, counts, compaction */
     UW_STAGE_PLAIN_OFFSETS = 2,  /* k_offsets: chunk offsets */
     UW_STAGE_PLAIN_EMIT = 3,     /* k_emit: vertices and indices */
-    UW_STAGE_BATCH_SETUP = 4,    /* k_prep + k_place: z tables and queue order */
+    UW_STAGE_BATCH_SETUP = 4,    /* k_prep (+ k_place with the bulk path): z tables, queue order */
     UW_STAGE_VOLUME_COUNT = 5,   /* k_vol_count */
     UW_STAGE_VOLUME_SCAN = 6,    /* k_vol_scan_* */
     UW_STAGE_VOLUME_EMIT = 7,    /* k_vol_emit */
