@@ -87,9 +87,12 @@ def test_batched_unpacked_reservations():
 
 
 def test_batched_queue_setup_forms():
-    # Queue set-up at small and large launch sizes (20000 chunks: 79 ordering workgroups, more than
-    # one batch of histogram loads in the placement kernel), alternating on one context so nothing
-    # of a launch's set-up state leaks into the next; each equal to the plain path chunk by chunk.
+    # Queue set-up at small and large launch sizes without the bulk path, where the ordering
+    # workgroups place their own chunks into an interleaved queue (20000 chunks: 79 workgroups, the
+    # last with 32 chunks; 300: two, the last with 44), alternating on one context so nothing of a
+    # launch's set-up state leaks into the next; each equal to the plain path chunk by chunk. The
+    # placement kernel's batched histogram loads (bulk launches of more than 4096 chunks) run in
+    # tests/test_gpu_fullsize.py.
     from underwaterworld_amd import Perlin, build_chunks_batched
     perlin = Perlin(PR.preset("chunks12"))
     big = [(x, y, z) for z in (-1, 0) for y in range(-50, 50) for x in range(-50, 50)]
