@@ -115,7 +115,7 @@ __device__ __forceinline__ int order_bucket(const uw_field& f, const uint8_t* P,
 // would spend ~5 us on the bucket estimates of 4096 chunks alone):
 //  k_prep:  workgroups [0, nest) take kPrepThreads chunks each: bucket estimate, rank within the
 //           bucket among the workgroup's chunks (LDS atomic), the workgroup's bucket histogram;
-//           the other workgroups write the chunks' z tables, one thread per record.
+//           the other workgroups write the chunks' z tables, one thread per record (ztab_block).
 //  k_place: workgroup j places the chunks of estimate workgroup j: bucket offsets from all
 //           histograms (total over all workgroups, plus the part of workgroups before j), then
 //           slot = offset + rank. Workgroup 0 also resets the totals and the range counters, and
@@ -276,6 +276,43 @@ __device__ __forceinline__ void estimate_block(const uw_field& f, const uw_table
 
 static_assert(kPrepThreads == 256 && kPrepThreads * 3 >= kOrderBuckets, "three buckets per thread in k_place");
 
+// The z records g0 .. g0 + kPrepThreads - 1 (one per thread; the layout of uw_ztab_record, so
+// consecutive records are contiguous), staged in LDS at a pitch of rw + 1 words (no bank aliasing)
+// and written as one block with 16-byte stores. A thread writing its own record with 4-byte stores
+// at a stride of rw words touched a separate cache line with every lane of every store. Records
+// uw_ztab_record leaves unwritten (invalid chunks, past a chunk's last sample) are written as zeros.
+__device__ __forceinline__ void ztab_block(const uw_field& f, const int32_t* __restrict__ coords,
+                                           const int32_t* __restrict__ res, int32_t nchunks, int32_t stride,
+                                           unsigned char* __restrict__ out, uint64_t g0, uint32_t* stage) {
+    const int tid = threadIdx.x;
+    const int32_t rw = uw_zrec_words(f.octaves), pitch = rw + 1;
+    const uint32_t per = uint32_t(stride) + 1u;
+    const uint64_t total = uint64_t(nchunks) * per;
+    const uint64_t g = g0 + uint64_t(tid);
+    uint32_t* rec = stage + tid * pitch;
+    bool filled = false;
+    if (g < total) {
+        const int32_t c = int32_t(g / per), i = int32_t(g - uint64_t(c) * per);
+        const int32_t n = res[c];
+        if (n >= 1 && n + 1 <= stride && i <= n + 1) {
+            uw_fill_zrec(f, coords[3 * c + 2], n + 1, float(f.extent) / float(n), i, rec);
+            filled = true;
+        }
+    }
+    if (!filled)
+        for (int32_t w = 0; w < rw; ++w) rec[w] = 0u;
+    __syncthreads();
+    if (g0 >= total) return;  // (block-uniform)
+    const uint32_t nrec = uint32_t(min(uint64_t(kPrepThreads), total - g0));
+    const uint32_t nq = nrec * uint32_t(rw) / 4u;  // 16-byte words of the block (rw is a multiple of 4)
+    uint4* dst = reinterpret_cast<uint4*>(out + g0 * uint64_t(rw) * 4u);
+    for (uint32_t q = uint32_t(tid); q < nq; q += kPrepThreads) {
+        const uint32_t wd = 4u * q, r = wd / uint32_t(rw), w = wd - r * uint32_t(rw);
+        const uint32_t* s = stage + r * uint32_t(pitch) + w;
+        dst[q] = make_uint4(s[0], s[1], s[2], s[3]);
+    }
+}
+
 __global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const uw_tables* __restrict__ tab,
                                                        const int32_t* __restrict__ coords,
                                                        const int32_t* __restrict__ res, int32_t nchunks,
@@ -287,8 +324,8 @@ __global__ void __launch_bounds__(kPrepThreads) k_prep(uw_field f, const uw_tabl
         estimate_block(f, tab, coords, res, nchunks, keys, hist, self_placed ? &pa : nullptr, nest);
         return;
     }
-    const uint64_t g = uint64_t(blockIdx.x - uint32_t(nest)) * kPrepThreads + threadIdx.x;
-    uw_ztab_record(f, coords, res, 0, nchunks, stride, ztab, g);
+    extern __shared__ uint32_t zstage[];  // kPrepThreads records, pitch uw_zrec_words(O) + 1
+    ztab_block(f, coords, res, nchunks, stride, ztab, uint64_t(blockIdx.x - uint32_t(nest)) * kPrepThreads, zstage);
 }
 
 __global__ void __launch_bounds__(kPrepThreads) k_place(const int32_t* __restrict__ coords,
@@ -1200,7 +1237,9 @@ int launch_batched(uw_ctx* ctx, const BatchLaunch& bl, const int32_t* coords, co
     if (int rc = uw_prof_start(ctx, st, UW_STAGE_BATCH_SETUP)) return rc;
     // a launch of at most kPrepThreads chunks, or one without the bulk path, is placed by its
     // estimate workgroups themselves
-    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks) + uint32_t(nest)), dim3(kPrepThreads), 0, st, a.f, a.tab, coords,
+    const size_t zstage_bytes = size_t(kPrepThreads) * size_t(uw_zrec_words(a.f.octaves) + 1) * sizeof(uint32_t);
+    hipLaunchKernelGGL(k_prep, dim3(uint32_t(zblocks) + uint32_t(nest)), dim3(kPrepThreads), zstage_bytes, st, a.f,
+                       a.tab, coords,
                        res, nchunks, keys, hist, nest, a.zstride, ztab, pa);
     if (nest > 1 && pa.bulk_bucket != 0u)
         hipLaunchKernelGGL(k_place, dim3(uint32_t(nest)), dim3(kPrepThreads), 0, st, coords, res, nchunks, keys, hist,
